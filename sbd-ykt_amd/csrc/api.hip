@@ -1,0 +1,888 @@
+// C-ABI of libsbdg and the device-resident Davidson driver.
+//
+// The driver follows Davidson(...,max_time) (tpb/davidson.h:661-924): lowest root,
+// diagonal preconditioner on the normalised residual with the 1e-12 regularisation,
+// Gram-Schmidt against the basis, convergence on |HW - E W| < eps, the max_time guard.
+// Differences, all numerically neutral: Gram-Schmidt is classical and applied twice
+// (each pass one multi-dot and one update) instead of one modified pass; at a restart
+// the kept Ritz vectors carry their sigma (HV rotated with the same coefficients), so
+// the sigma of the restarted vector is not recomputed; restart_keep > 1 keeps that many
+// Ritz vectors (thick restart).  Every vector stays in HBM; the host sees only the
+// subspace scalars.
+//
+// Mixed precision (options.precision = 1): basis vectors, Gram-Schmidt, the subspace
+// eigensolve and all residual bookkeeping stay FP64; the sigma of each new basis vector
+// is built in FP32 (FP32 vectors, integrals and per-row accumulation).  Every refresh_k
+// steps the current Ritz vector x becomes the first basis vector and its sigma is rebuilt
+// in FP64, which gives the exact Rayleigh quotient and residual (used for convergence
+// and reported) and an exact first row/column of the projected matrix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "sbdg_internal.h"
+
+using namespace sbdg;
+
+namespace sbdg {
+void multi_dot_impl(const double* X, int64_t ldx, int m, const double* y, int64_t n, double* out, double* part,
+                    hipStream_t st);
+void multi_dot_impl(const double* X, int64_t ldx, int m, const float* y, int64_t n, double* out, double* part,
+                    hipStream_t st);
+template <class TH>
+void ritz_combine_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, int m, const double* u,
+                       const double* theta, double* W, double* R, int64_t n, double* out, double* part,
+                       hipStream_t st);
+void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st);
+void precond_impl(const double* R, const double* rn2, const double* hii, const double* theta, double* t, int64_t n,
+                  hipStream_t st);
+void gs_impl(const double* V, int64_t ldv, int m, const double* s, double* t, int64_t n, double* out, double* part,
+             hipStream_t st);
+template <class T>
+void rotate_impl(T* X, int64_t ldx, int m, const double* U, int ld, int mk, int64_t n, hipStream_t st);
+}  // namespace sbdg
+
+namespace {
+constexpr int MAXSUB = 48;
+constexpr size_t LDS_BUDGET = 80 * 1024;
+}  // namespace
+
+struct sbdg_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  sbdg_options opt{};
+  bool have_ints = false, have_strs = false, built = false, solved = false, have_rdm = false,
+       have_dens = false, have_user_vec = false;
+  int L = 0;
+  double core = 0.0;
+  std::vector<double> h1, eri;  // eri full L^4
+  std::vector<uint64_t> a, b;
+  std::vector<double> user_vec;
+  DBuf<double> d_h1, d_eri, d_hii;
+  DBuf<float> d_eri32, d_hii32;
+  DBuf<uint64_t> d_a, d_b;
+  Table Sa, Da, Sb, Db;
+  SpmmTable Ma, Mb;
+  AbTable ab64, ab32;
+  // solver state
+  DBuf<double> V, HV, W, R, T, HX, scr64, small;
+  DBuf<float> HV32, c32, s32, scr32;
+  DBuf<double> d_one, d_two, d_dens;
+  sbdg_stats stats{};
+  std::vector<double> traj_t, traj_e;
+  double energy = 0.0;
+};
+
+namespace {
+
+int fail(sbdg_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+template <class F>
+int guard(sbdg_ctx* c, F&& f) {
+  if (!c) return SBDG_ERR_ARG;
+  try {
+    c->err.clear();
+    SBDG_CHECK(hipSetDevice(c->device));
+    return f();
+  } catch (const Error& e) {
+    return fail(c, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(c, SBDG_ERR_HIP, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(c, SBDG_ERR_ARG, e.what());
+  }
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int64_t choose_chunk(int64_t nb, int L, int elem) {
+  const int64_t budget = (int64_t)(LDS_BUDGET / elem) - (int64_t)L * L;
+  int64_t ch = std::max<int64_t>(256, budget);
+  return std::min<int64_t>(std::max<int64_t>(nb, 1), ch);
+}
+
+SigmaPlan plan(sbdg_ctx* c, bool fp32) {
+  SigmaPlan P;
+  P.na = (int64_t)c->a.size();
+  P.nb = (int64_t)c->b.size();
+  P.L = c->L;
+  P.a = c->d_a.p;
+  P.b = c->d_b.p;
+  P.ma = &c->Ma;
+  P.mb = &c->Mb;
+  P.sa = &c->Sa;
+  P.ab = fp32 ? &c->ab32 : &c->ab64;
+  P.eri = c->d_eri.p;
+  P.eri32 = c->d_eri32.p;
+  P.hii = c->d_hii.p;
+  P.hii32 = c->d_hii32.p;
+  return P;
+}
+
+void build(sbdg_ctx* c) {
+  if (!c->have_ints) throw Error(SBDG_ERR_STATE, "integrals not set");
+  if (!c->have_strs) throw Error(SBDG_ERR_STATE, "strings not set");
+  const double t0 = now();
+  const int L = c->L;
+  const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+  const int64_t L4 = (int64_t)L * L * L * L;
+  c->d_h1.alloc(L * L);
+  c->d_eri.alloc(L4);
+  c->d_eri32.alloc(L4);
+  SBDG_CHECK(hipMemcpyAsync(c->d_h1.p, c->h1.data(), sizeof(double) * L * L, hipMemcpyHostToDevice, c->st));
+  SBDG_CHECK(hipMemcpyAsync(c->d_eri.p, c->eri.data(), sizeof(double) * L4, hipMemcpyHostToDevice, c->st));
+  cast_d2f(c->d_eri.p, c->d_eri32.p, L4, c->st);
+  c->d_a.alloc(std::max<int64_t>(na, 1));
+  c->d_b.alloc(std::max<int64_t>(nb, 1));
+  SBDG_CHECK(hipMemcpyAsync(c->d_a.p, c->a.data(), sizeof(uint64_t) * na, hipMemcpyHostToDevice, c->st));
+  SBDG_CHECK(hipMemcpyAsync(c->d_b.p, c->b.data(), sizeof(uint64_t) * nb, hipMemcpyHostToDevice, c->st));
+  build_table(c->d_a.p, na, L, c->d_h1.p, c->d_eri.p, c->Sa, c->Da, c->st);
+  build_table(c->d_b.p, nb, L, c->d_h1.p, c->d_eri.p, c->Sb, c->Db, c->st);
+  build_spmm(c->Sa, c->Da, c->Ma, c->st);
+  build_spmm(c->Sb, c->Db, c->Mb, c->st);
+  build_ab(c->Sb, L, choose_chunk(nb, L, 8), c->ab64, c->st);
+  build_ab(c->Sb, L, choose_chunk(nb, L, 4), c->ab32, c->st);
+  c->d_hii.alloc(std::max<int64_t>(N, 1));
+  c->d_hii32.alloc(std::max<int64_t>(N, 1));
+  compute_hii(c->d_a.p, na, c->d_b.p, nb, L, c->core, c->d_h1.p, c->d_eri.p, c->d_hii.p, c->st);
+  cast_d2f(c->d_hii.p, c->d_hii32.p, N, c->st);
+  SBDG_CHECK(hipStreamSynchronize(c->st));
+  c->built = true;
+  c->solved = c->have_rdm = c->have_dens = false;
+  c->stats = sbdg_stats{};
+  c->stats.t_tables = now() - t0;
+  c->stats.nnz[0] = c->Sa.nnz;
+  c->stats.nnz[1] = c->Da.nnz;
+  c->stats.nnz[2] = c->Sb.nnz;
+  c->stats.nnz[3] = c->Db.nnz;
+  c->stats.terms = N + (c->Sa.nnz + c->Da.nnz) * nb + (c->Sb.nnz + c->Db.nnz) * na + c->Sa.nnz * c->Sb.nnz;
+}
+
+void ensure_built(sbdg_ctx* c) {
+  if (!c->built) build(c);
+}
+
+// device scratch for one sigma of precision T
+void sigma64(sbdg_ctx* c, const double* x, double* y, hipEvent_t* ev = nullptr) {
+  const int64_t N = (int64_t)c->a.size() * c->b.size();
+  c->scr64.alloc(std::max<int64_t>(2 * N, 1));
+  sigma_apply<double>(plan(c, false), x, y, c->scr64.p, c->st, ev);
+}
+
+void sigma32(sbdg_ctx* c, const float* x, float* y, hipEvent_t* ev = nullptr) {
+  const int64_t N = (int64_t)c->a.size() * c->b.size();
+  c->scr32.alloc(std::max<int64_t>(2 * N, 1));
+  sigma_apply<float>(plan(c, true), x, y, c->scr32.p, c->st, ev);
+}
+
+struct Small {  // device scalars carved from one buffer
+  double *H, *U, *E, *dots, *norms, *part;
+};
+
+Small carve(sbdg_ctx* c) {
+  const size_t ms = MAXSUB * MAXSUB;
+  c->small.alloc(4 * ms + 4 * MAXSUB + 64 + 1024 * (MAXSUB + 2));
+  Small s;
+  s.H = c->small.p;
+  s.U = s.H + ms;
+  s.E = s.U + ms;
+  s.dots = s.E + MAXSUB;
+  s.norms = s.dots + MAXSUB;
+  s.part = s.norms + 64;
+  return s;
+}
+
+void initial_vector(sbdg_ctx* c, double* W) {
+  const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+  std::vector<double> h(N, 0.0);
+  if (c->opt.init == 2) {
+    if (!c->have_user_vec) throw Error(SBDG_ERR_STATE, "init=2 but no vector set");
+    h = c->user_vec;
+  } else if (c->opt.init == 1) {
+    std::mt19937_64 g(c->opt.seed);
+    std::uniform_real_distribution<double> d(-1.0, 1.0);
+    for (auto& x : h) x = d(g);
+  } else {
+    h[0] = 1.0;  // BasisInitVector (tpb/davidson.h:130-133): the first determinant
+  }
+  double nn = 0.0;
+  for (double x : h) nn += x * x;
+  if (!(nn > 0.0) || !std::isfinite(nn)) throw Error(SBDG_ERR_ARG, "initial vector has zero or non-finite norm");
+  const double f = 1.0 / std::sqrt(nn);
+  for (auto& x : h) x *= f;
+  SBDG_CHECK(hipMemcpyAsync(c->T.p, h.data(), sizeof(double) * N, hipMemcpyHostToDevice, c->st));
+  sign_transform(c->d_a.p, na, c->d_b.p, nb, c->T.p, W, c->st);  // reference phases -> separated
+  SBDG_CHECK(hipStreamSynchronize(c->st));
+}
+
+// t = precondition(R) then classical Gram-Schmidt twice against V[0..m), normalised
+// into V[m].  Returns |t| before the final normalisation.
+double expand(sbdg_ctx* c, Small& s, int m, int64_t N, const double* theta_dev, const double* rn2_dev) {
+  double* vm = c->V.p + (int64_t)m * N;
+  precond_impl(c->R.p, rn2_dev, c->d_hii.p, theta_dev, vm, N, c->st);
+  double nt = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    multi_dot_impl(c->V.p, N, m, vm, N, s.dots, s.part, c->st);
+    gs_impl(c->V.p, N, m, s.dots, vm, N, s.norms + 8, s.part, c->st);
+  }
+  SBDG_CHECK(hipMemcpyAsync(&nt, s.norms + 8, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  SBDG_CHECK(hipStreamSynchronize(c->st));
+  if (!std::isfinite(nt)) throw Error(SBDG_ERR_NUMERIC, "non-finite correction vector");
+  if (nt <= 0.0) return 0.0;
+  scale_impl(vm, s.norms + 8, N, c->st);
+  return std::sqrt(nt);
+}
+
+void davidson(sbdg_ctx* c) {
+  const sbdg_options& o = c->opt;
+  const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+  const bool mixed = o.precision == 1;
+  const int nbmax = std::max(2, std::min(o.max_nb, MAXSUB));
+  const int keep = std::max(1, std::min(o.restart_keep, nbmax - 1));
+  const int max_steps = std::max(1, o.max_it) * nbmax;
+  c->V.alloc((size_t)nbmax * N);
+  c->W.alloc(N);
+  c->R.alloc(N);
+  c->T.alloc(N);
+  c->HX.alloc(N);
+  if (mixed) {
+    c->HV32.alloc((size_t)nbmax * N);
+    c->c32.alloc(N);
+  } else {
+    c->HV.alloc((size_t)nbmax * N);
+  }
+  Small s = carve(c);
+  // device scalar slots: norms[0] |W|^2, [1] |R|^2 (combine), [2] theta and [3] |R|^2 used
+  // by the next expansion, [6] the constant 1, [8] |t|^2
+  double* th_exp = s.norms + 2;
+  double* rr_exp = s.norms + 3;
+  double* one_dev = s.norms + 6;
+  std::vector<double> Hs((size_t)MAXSUB * MAXSUB, 0.0), E(MAXSUB), dots(MAXSUB);
+  hipEvent_t e0, e1;
+  SBDG_CHECK(hipEventCreate(&e0));
+  SBDG_CHECK(hipEventCreate(&e1));
+  auto timed = [&](auto&& f) {
+    SBDG_CHECK(hipEventRecord(e0, c->st));
+    f();
+    SBDG_CHECK(hipEventRecord(e1, c->st));
+    SBDG_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SBDG_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    c->stats.t_sigma += 1e-3 * ms;
+  };
+  auto h2d = [&](double* d, double x) {
+    SBDG_CHECK(hipMemcpyAsync(d, &x, sizeof(double), hipMemcpyHostToDevice, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+  };
+
+  initial_vector(c, c->V.p);
+  h2d(one_dev, 1.0);
+  const double tstart = now();
+  c->traj_t.clear();
+  c->traj_e.clear();
+  int h = 0, v = 1, steps = 0, cycle = 0, since = 0;
+  double theta = 0.0, rnorm = 0.0;
+  bool converged = false, stop = false;
+  std::vector<double> step_times;
+
+  auto log_step = [&](bool exact) {
+    const double el = now() - tstart;
+    c->stats.iterations = ++steps;
+    c->stats.ritz = theta;
+    c->stats.resid = rnorm;
+    if (exact) {
+      c->traj_t.push_back(el);
+      c->traj_e.push_back(theta);
+    }
+    if (o.verbose)
+      printf(" Davidson step %d (basis %d, %s): E = %.15f |r| = %.3e  t = %.3f s\n", steps, h,
+             exact ? "fp64" : "fp32 sigma", theta, rnorm, el);
+    fflush(stdout);
+    return el;
+  };
+  // projected-matrix eigenpair and Ritz vector/residual over the first m vectors
+  auto ritz = [&](int m) {
+    SBDG_CHECK(hipMemcpyAsync(s.H, Hs.data(), sizeof(double) * MAXSUB * MAXSUB, hipMemcpyHostToDevice, c->st));
+    syev_device(s.H, m, MAXSUB, s.E, s.U, c->st);
+    if (mixed)
+      ritz_combine_impl<float>(c->V.p, N, c->HV32.p, N, m, s.U, s.E, c->W.p, c->R.p, N, s.norms, s.part, c->st);
+    else
+      ritz_combine_impl<double>(c->V.p, N, c->HV.p, N, m, s.U, s.E, c->W.p, c->R.p, N, s.norms, s.part, c->st);
+    double nr[2];
+    SBDG_CHECK(hipMemcpyAsync(nr, s.norms, sizeof(double) * 2, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipMemcpyAsync(E.data(), s.E, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    if (!std::isfinite(nr[0]) || !std::isfinite(nr[1]) || !std::isfinite(E[0]))
+      throw Error(SBDG_ERR_NUMERIC, "non-finite Ritz value or residual");
+    scale_impl(c->W.p, s.norms, N, c->st);  // Normalize(W) (davidson.h:818-819)
+    SBDG_CHECK(hipMemcpyAsync(th_exp, s.E, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    SBDG_CHECK(hipMemcpyAsync(rr_exp, s.norms + 1, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    theta = E[0];
+    rnorm = std::sqrt(nr[1]);
+  };
+  // mixed: FP64 sigma of basis vector 0 (= current Ritz vector): exact row/column 0 of
+  // the projected matrix, exact Rayleigh quotient and residual
+  auto refresh = [&]() {
+    timed([&] { sigma64(c, c->V.p, c->HX.p); });
+    c->stats.sigma_fp64++;
+    multi_dot_impl(c->V.p, N, v, c->HX.p, N, s.dots, s.part, c->st);
+    SBDG_CHECK(hipMemcpyAsync(dots.data(), s.dots, sizeof(double) * v, hipMemcpyDeviceToHost, c->st));
+    cast_d2f(c->HX.p, c->HV32.p, N, c->st);
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    for (int k = 0; k < v; ++k) Hs[k] = Hs[MAXSUB * k] = dots[k];
+    theta = dots[0];
+    h2d(th_exp, theta);
+    ritz_combine_impl<double>(c->V.p, N, c->HX.p, N, 1, one_dev, th_exp, c->T.p, c->R.p, N, s.norms, s.part, c->st);
+    double nr[2];
+    SBDG_CHECK(hipMemcpyAsync(nr, s.norms, sizeof(double) * 2, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipMemcpyAsync(rr_exp, s.norms + 1, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    if (!std::isfinite(nr[1]) || !std::isfinite(theta)) throw Error(SBDG_ERR_NUMERIC, "non-finite residual");
+    rnorm = std::sqrt(nr[1]);
+    SBDG_CHECK(hipMemcpyAsync(c->W.p, c->V.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->st));
+    since = 0;
+  };
+  auto check_time = [&](double el, double ts) {
+    step_times.push_back(now() - ts);
+    double avg = 0.0;
+    for (double x : step_times) avg += x;
+    avg /= step_times.size();
+    return el + avg > o.max_time;  // davidson.h:880-903
+  };
+  auto expand_or_stop = [&]() {
+    if (v >= nbmax) return;
+    const double nt = expand(c, s, v, N, th_exp, rr_exp);
+    if (nt == 0.0) {
+      stop = true;  // the residual lies in the span of the basis
+      return;
+    }
+    ++v;
+  };
+
+  if (mixed) {
+    h = 1;
+    const double ts = now();
+    refresh();
+    const double el = log_step(true);
+    if (rnorm < o.eps) converged = true;
+    else if (check_time(el, ts)) stop = true;
+    else expand_or_stop();
+  }
+  double prev_exact = theta;
+  while (!converged && !stop && steps < max_steps) {
+    const double ts = now();
+    // sigma of the newest basis vector and column h of the projected matrix
+    if (mixed) {
+      timed([&] {
+        cast_d2f(c->V.p + (int64_t)h * N, c->c32.p, N, c->st);
+        sigma32(c, c->c32.p, c->HV32.p + (int64_t)h * N);
+      });
+      c->stats.sigma_fp32++;
+      multi_dot_impl(c->V.p, N, h + 1, c->HV32.p + (int64_t)h * N, N, s.dots, s.part, c->st);
+      SBDG_CHECK(hipMemcpyAsync(dots.data(), s.dots, sizeof(double) * (h + 1), hipMemcpyDeviceToHost, c->st));
+      multi_dot_impl(c->V.p + (int64_t)h * N, N, 1, c->HX.p, N, s.norms + 10, s.part, c->st);
+      SBDG_CHECK(hipMemcpyAsync(&dots[0], s.norms + 10, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    } else {
+      timed([&] { sigma64(c, c->V.p + (int64_t)h * N, c->HV.p + (int64_t)h * N); });
+      c->stats.sigma_fp64++;
+      multi_dot_impl(c->V.p, N, h + 1, c->HV.p + (int64_t)h * N, N, s.dots, s.part, c->st);
+      SBDG_CHECK(hipMemcpyAsync(dots.data(), s.dots, sizeof(double) * (h + 1), hipMemcpyDeviceToHost, c->st));
+    }
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    for (int k = 0; k <= h; ++k) Hs[k + MAXSUB * h] = Hs[h + MAXSUB * k] = dots[k];
+    ++h;
+    ++since;
+    ritz(h);
+    if (!mixed) {
+      const double el = log_step(true);
+      if (rnorm < o.eps) {
+        converged = true;
+        break;
+      }
+      if (check_time(el, ts)) break;
+      if (v == nbmax) {
+        // restart (davidson.h:912-916) keeping `keep` Ritz vectors with their sigma
+        if (cycle + 1 >= std::max(1, o.max_it)) break;
+        ++cycle;
+        rotate_impl<double>(c->V.p, N, h, s.U, MAXSUB, keep, N, c->st);
+        rotate_impl<double>(c->HV.p, N, h, s.U, MAXSUB, keep, N, c->st);
+        std::fill(Hs.begin(), Hs.end(), 0.0);
+        for (int k = 0; k < keep; ++k) Hs[k + MAXSUB * k] = E[k];
+        v = h = keep;
+      }
+      expand_or_stop();
+    } else {
+      const bool do_refresh = since >= std::max(1, o.refresh_k) || v == nbmax;
+      if (!do_refresh) {
+        const double el = log_step(false);
+        if (check_time(el, ts)) break;
+        expand_or_stop();
+        continue;
+      }
+      // the Ritz vector becomes basis vector 0 (thick restart to `keep` vectors)
+      rotate_impl<double>(c->V.p, N, h, s.U, MAXSUB, keep, N, c->st);
+      rotate_impl<float>(c->HV32.p, N, h, s.U, MAXSUB, keep, N, c->st);
+      SBDG_CHECK(hipMemcpyAsync(c->V.p, c->W.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->st));
+      std::fill(Hs.begin(), Hs.end(), 0.0);
+      for (int k = 1; k < keep; ++k) Hs[k + MAXSUB * k] = E[k];
+      v = h = keep;
+      ++cycle;
+      refresh();
+      const double el = log_step(true);
+      if (rnorm < o.eps) {
+        converged = true;
+        break;
+      }
+      if (o.target_de > 0.0 && std::fabs(theta - prev_exact) < o.target_de) {
+        converged = true;
+        break;
+      }
+      prev_exact = theta;
+      if (check_time(el, ts)) break;
+      expand_or_stop();
+    }
+  }
+  c->stats.converged = converged ? 1 : 0;
+  c->stats.t_davidson = now() - tstart;
+  SBDG_CHECK(hipEventDestroy(e0));
+  SBDG_CHECK(hipEventDestroy(e1));
+  // final energy: one more FP64 sigma of the normalised W, then <W|HW> (sbdiag.h:250-273)
+  const double tf = now();
+  sigma64(c, c->W.p, c->HX.p);
+  c->stats.sigma_fp64++;
+  multi_dot_impl(c->W.p, N, 1, c->HX.p, N, s.dots, s.part, c->st);
+  double e = 0.0;
+  SBDG_CHECK(hipMemcpyAsync(&e, s.dots, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  SBDG_CHECK(hipStreamSynchronize(c->st));
+  c->energy = e;
+  c->stats.energy = e;
+  c->stats.t_final = now() - tf;
+  c->solved = true;
+}
+
+void rdm_or_density(sbdg_ctx* c) {
+  const double t0 = now();
+  const int L = c->L;
+  const int64_t na = c->a.size(), nb = c->b.size();
+  if (c->opt.do_rdm) {
+    c->d_one.alloc(2 * L * L);
+    c->d_two.alloc((size_t)4 * L * L * L * L);
+    RdmPlan P;
+    P.na = na;
+    P.nb = nb;
+    P.L = L;
+    P.a = c->d_a.p;
+    P.b = c->d_b.p;
+    P.sa = &c->Sa;
+    P.da = &c->Da;
+    P.sb = &c->Sb;
+    P.db = &c->Db;
+    P.ab = &c->ab64;
+    rdm_compute(P, c->W.p, c->d_one.p, c->d_two.p, c->st);
+    c->have_rdm = true;
+  }
+  c->d_dens.alloc(2 * L);
+  c->small.alloc(std::max<size_t>(c->small.n, (size_t)(na + nb + 16)));
+  DBuf<double> part;
+  part.alloc(na + nb + 1);
+  density(c->W.p, c->d_a.p, na, c->d_b.p, nb, L, c->d_dens.p, part.p, c->st);
+  SBDG_CHECK(hipStreamSynchronize(c->st));
+  c->have_dens = true;
+  c->stats.t_rdm = now() - t0;
+}
+
+}  // namespace
+
+extern "C" {
+
+sbdg_ctx* sbdg_create(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  auto* c = new sbdg_ctx();
+  c->device = device;
+  if (hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return nullptr;
+  }
+  sbdg_default_options(&c->opt);
+  return c;
+}
+
+void sbdg_destroy(sbdg_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->st);
+  (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char* sbdg_last_error(const sbdg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sbdg_default_options(sbdg_options* o) {
+  if (!o) return SBDG_ERR_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->max_it = 1;
+  o->max_nb = 10;
+  o->eps = 1.0e-4;
+  o->max_time = 86400.0;
+  o->init = 0;
+  o->do_rdm = 0;
+  o->ratio = 0.0;
+  o->threshold = 0.01;
+  o->restart_keep = 1;
+  o->precision = 0;
+  o->refresh_k = 4;
+  o->target_de = 0.0;
+  o->verbose = 0;
+  o->seed = 12345;
+  return SBDG_OK;
+}
+
+int sbdg_set_integrals(sbdg_ctx* c, int norb, double core, const double* h1, const double* eri, int layout) {
+  return guard(c, [&]() {
+    if (norb < 1 || norb > 64) return fail(c, SBDG_ERR_ARG, "norb must be in 1..64");
+    if (!h1 || !eri) return fail(c, SBDG_ERR_ARG, "null integral pointer");
+    const int L = norb;
+    const int64_t L4 = (int64_t)L * L * L * L;
+    c->L = L;
+    c->core = core;
+    c->h1.assign(h1, h1 + L * L);
+    c->eri.assign(L4, 0.0);
+    if (layout == 0) {
+      std::memcpy(c->eri.data(), eri, sizeof(double) * L4);
+    } else if (layout == 1) {
+      // twoInt::store (integrals.h:24-67): slot of (ij|kl) = a(a+1)/2+b, a=max(ij,kl),
+      // b=min(ij,kl), ij = max(i,j)(max(i,j)+1)/2 + min(i,j)
+      auto pr = [](int64_t i, int64_t j) { return i > j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
+      for (int p = 0; p < L; ++p)
+        for (int q = 0; q < L; ++q)
+          for (int r = 0; r < L; ++r)
+            for (int s_ = 0; s_ < L; ++s_) {
+              const int64_t ij = pr(p, q), kl = pr(r, s_);
+              const int64_t aa = std::max(ij, kl), bb = std::min(ij, kl);
+              c->eri[(((int64_t)p * L + q) * L + r) * L + s_] = eri[aa * (aa + 1) / 2 + bb];
+            }
+    } else {
+      return fail(c, SBDG_ERR_ARG, "eri_layout must be 0 or 1");
+    }
+    for (double x : c->h1)
+      if (!std::isfinite(x)) return fail(c, SBDG_ERR_ARG, "non-finite one-electron integral");
+    for (double x : c->eri)
+      if (!std::isfinite(x)) return fail(c, SBDG_ERR_ARG, "non-finite two-electron integral");
+    c->have_ints = true;
+    c->built = c->solved = c->have_rdm = c->have_dens = false;
+    return SBDG_OK;
+  });
+}
+
+static std::string check_set(const uint64_t* s, int64_t n, int L, const char* name) {
+  if (n < 1) return std::string(name) + " string set is empty";
+  const int pc = __builtin_popcountll(s[0]);
+  std::vector<uint64_t> v(s, s + n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (L < 64 && (s[i] >> L) != 0) return std::string(name) + " string has orbitals beyond norb";
+    if (__builtin_popcountll(s[i]) != pc) return std::string(name) + " strings differ in electron count";
+  }
+  std::sort(v.begin(), v.end());
+  if (std::adjacent_find(v.begin(), v.end()) != v.end()) return std::string(name) + " strings are not unique";
+  return "";
+}
+
+int sbdg_set_strings(sbdg_ctx* c, const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb) {
+  return guard(c, [&]() {
+    if (!c->have_ints) return fail(c, SBDG_ERR_STATE, "set integrals before strings");
+    if (!a || !b) return fail(c, SBDG_ERR_ARG, "null string pointer");
+    std::string m = check_set(a, na, c->L, "alpha");
+    if (m.empty()) m = check_set(b, nb, c->L, "beta");
+    if (!m.empty()) return fail(c, SBDG_ERR_ARG, m);
+    if (nb > 65536) return fail(c, SBDG_ERR_LIMIT, "beta string set beyond 65536 strings");
+    if (na > (int64_t)INT32_MAX) return fail(c, SBDG_ERR_LIMIT, "alpha string set beyond 2^31 strings");
+    c->a.assign(a, a + na);
+    c->b.assign(b, b + nb);
+    c->have_strs = true;
+    c->built = c->solved = c->have_rdm = c->have_dens = c->have_user_vec = false;
+    return SBDG_OK;
+  });
+}
+
+int sbdg_set_options(sbdg_ctx* c, const sbdg_options* o) {
+  return guard(c, [&]() {
+    if (!o) return fail(c, SBDG_ERR_ARG, "null options");
+    if (o->max_nb < 1 || o->max_nb > MAXSUB) return fail(c, SBDG_ERR_ARG, "max_nb must be in 1..48");
+    if (o->max_it < 1) return fail(c, SBDG_ERR_ARG, "max_it must be >= 1");
+    if (o->precision != 0 && o->precision != 1) return fail(c, SBDG_ERR_ARG, "precision must be 0 or 1");
+    if (o->init < 0 || o->init > 2) return fail(c, SBDG_ERR_ARG, "init must be 0, 1 or 2");
+    if (!(o->eps >= 0.0)) return fail(c, SBDG_ERR_ARG, "eps must be >= 0");
+    c->opt = *o;
+    return SBDG_OK;
+  });
+}
+
+int sbdg_set_vector(sbdg_ctx* c, const double* w) {
+  return guard(c, [&]() {
+    if (!c->have_strs) return fail(c, SBDG_ERR_STATE, "strings not set");
+    if (!w) return fail(c, SBDG_ERR_ARG, "null vector");
+    const int64_t N = (int64_t)c->a.size() * c->b.size();
+    c->user_vec.assign(w, w + N);
+    c->have_user_vec = true;
+    return SBDG_OK;
+  });
+}
+
+int sbdg_build(sbdg_ctx* c) {
+  return guard(c, [&]() {
+    build(c);
+    return SBDG_OK;
+  });
+}
+
+int sbdg_run(sbdg_ctx* c) {
+  return guard(c, [&]() {
+    ensure_built(c);
+    const int64_t N = (int64_t)c->a.size() * c->b.size();
+    c->T.alloc(N);
+    davidson(c);
+    rdm_or_density(c);
+    return SBDG_OK;
+  });
+}
+
+int sbdg_get_stats(const sbdg_ctx* c, sbdg_stats* st) {
+  if (!c || !st) return SBDG_ERR_ARG;
+  *st = c->stats;
+  return SBDG_OK;
+}
+
+int sbdg_get_energy(const sbdg_ctx* c, double* e, double* r, int* it) {
+  if (!c) return SBDG_ERR_ARG;
+  if (!c->solved) return SBDG_ERR_STATE;
+  if (e) *e = c->energy;
+  if (r) *r = c->stats.resid;
+  if (it) *it = c->stats.iterations;
+  return SBDG_OK;
+}
+
+int sbdg_get_trajectory(const sbdg_ctx* c, double* t, double* e, int cap, int* n) {
+  if (!c || !n) return SBDG_ERR_ARG;
+  const int m = (int)c->traj_t.size();
+  *n = m;
+  for (int k = 0; k < std::min(m, cap); ++k) {
+    if (t) t[k] = c->traj_t[k];
+    if (e) e[k] = c->traj_e[k];
+  }
+  return SBDG_OK;
+}
+
+int sbdg_get_vector(sbdg_ctx* c, double* w) {
+  return guard(c, [&]() {
+    if (!c->solved) return fail(c, SBDG_ERR_STATE, "no solution");
+    if (!w) return fail(c, SBDG_ERR_ARG, "null output");
+    const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+    sign_transform(c->d_a.p, na, c->d_b.p, nb, c->W.p, c->T.p, c->st);
+    SBDG_CHECK(hipMemcpyAsync(w, c->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_get_density(sbdg_ctx* c, double* d) {
+  return guard(c, [&]() {
+    if (!c->have_dens) return fail(c, SBDG_ERR_STATE, "no density");
+    SBDG_CHECK(hipMemcpy(d, c->d_dens.p, sizeof(double) * 2 * c->L, hipMemcpyDeviceToHost));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_compute_rdm(sbdg_ctx* c) {
+  return guard(c, [&]() {
+    if (!c->solved) return fail(c, SBDG_ERR_STATE, "no solution");
+    const int save = c->opt.do_rdm;
+    c->opt.do_rdm = 1;
+    rdm_or_density(c);
+    c->opt.do_rdm = save;
+    return SBDG_OK;
+  });
+}
+
+int sbdg_get_rdm(sbdg_ctx* c, double* one, double* two) {
+  return guard(c, [&]() {
+    if (!c->have_rdm) return fail(c, SBDG_ERR_STATE, "no RDMs (run with do_rdm=1 or call sbdg_compute_rdm)");
+    const int64_t L = c->L;
+    if (one) SBDG_CHECK(hipMemcpy(one, c->d_one.p, sizeof(double) * 2 * L * L, hipMemcpyDeviceToHost));
+    if (two) SBDG_CHECK(hipMemcpy(two, c->d_two.p, sizeof(double) * 4 * L * L * L * L, hipMemcpyDeviceToHost));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_rdm_backward(sbdg_ctx* c, double g, double* dh1, double* deri) {
+  return guard(c, [&]() {
+    if (!c->have_rdm) return fail(c, SBDG_ERR_STATE, "no RDMs");
+    const int64_t L = c->L, L4 = L * L * L * L;
+    DBuf<double> dh, de;
+    dh.alloc(L * L);
+    de.alloc(L4);
+    rdm_backward(c->d_one.p, c->d_two.p, (int)L, g, dh.p, de.p, c->st);
+    if (dh1) SBDG_CHECK(hipMemcpyAsync(dh1, dh.p, sizeof(double) * L * L, hipMemcpyDeviceToHost, c->st));
+    if (deri) SBDG_CHECK(hipMemcpyAsync(deri, de.p, sizeof(double) * L4, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_carryover(sbdg_ctx* c, double ratio, double threshold, uint64_t* out, int64_t cap, int64_t* count,
+                   double* truncated) {
+  return guard(c, [&]() {
+    if (!c->solved) return fail(c, SBDG_ERR_STATE, "no solution");
+    const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+    std::vector<double> w(N);
+    SBDG_CHECK(hipMemcpy(w.data(), c->W.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    // DiagonalReducedDensityMatrix (tpb/rdmat.h:11-58): row weights of |W|^2
+    std::vector<double> D(na, 0.0);
+    for (int64_t ia = 0; ia < na; ++ia)
+      for (int64_t ib = 0; ib < nb; ++ib) D[ia] += w[ia * nb + ib] * w[ia * nb + ib];
+    std::vector<int64_t> idx(na);
+    for (int64_t i = 0; i < na; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return D[x] > D[y]; });
+    int64_t kept;
+    double tw = 0.0;
+    if (ratio > 0.0) {
+      kept = std::min<int64_t>(na, (int64_t)(ratio * na));
+      double s_ = 0.0;
+      for (int64_t i = 0; i < kept; ++i) s_ += D[idx[i]];
+      tw = 1.0 - s_;
+    } else {
+      // CarryOverAlphaDet by threshold (tpb/rdmat.h:60-100): smallest prefix whose running
+      // weight passes 1 - threshold (upper_bound + 1)
+      const double target = 1.0 - threshold;
+      double s_ = 0.0;
+      kept = na;
+      for (int64_t i = 0; i < na; ++i) {
+        s_ += D[idx[i]];
+        if (s_ > target) {  // first running sum above the target, kept inclusively
+          kept = i + 1;
+          break;
+        }
+      }
+      kept = std::min<int64_t>(kept, na);
+    }
+    if (count) *count = kept;
+    if (truncated) *truncated = tw;
+    for (int64_t i = 0; i < std::min(kept, cap); ++i)
+      if (out) out[i] = c->a[idx[i]];
+    return SBDG_OK;
+  });
+}
+
+int sbdg_table_nnz(const sbdg_ctx* c, int which, int64_t* nnz) {
+  if (!c || !nnz || which < 0 || which > 3) return SBDG_ERR_ARG;
+  if (!c->built) return SBDG_ERR_STATE;
+  const Table* t[4] = {&c->Sa, &c->Da, &c->Sb, &c->Db};
+  *nnz = t[which]->nnz;
+  return SBDG_OK;
+}
+
+int sbdg_get_table(sbdg_ctx* c, int which, int64_t* rowptr, int32_t* col, uint32_t* exc, double* val) {
+  return guard(c, [&]() {
+    if (which < 0 || which > 3) return fail(c, SBDG_ERR_ARG, "which must be 0..3");
+    ensure_built(c);
+    const Table* t[4] = {&c->Sa, &c->Da, &c->Sb, &c->Db};
+    const Table& tb = *t[which];
+    if (rowptr) SBDG_CHECK(hipMemcpy(rowptr, tb.rowptr.p, sizeof(int64_t) * (tb.n + 1), hipMemcpyDeviceToHost));
+    if (tb.nnz) {
+      if (col) SBDG_CHECK(hipMemcpy(col, tb.col.p, sizeof(int32_t) * tb.nnz, hipMemcpyDeviceToHost));
+      if (exc) SBDG_CHECK(hipMemcpy(exc, tb.exc.p, sizeof(uint32_t) * tb.nnz, hipMemcpyDeviceToHost));
+      if (val) SBDG_CHECK(hipMemcpy(val, tb.val.p, sizeof(double) * tb.nnz, hipMemcpyDeviceToHost));
+    }
+    return SBDG_OK;
+  });
+}
+
+int sbdg_get_hii(sbdg_ctx* c, double* hii) {
+  return guard(c, [&]() {
+    ensure_built(c);
+    const int64_t N = (int64_t)c->a.size() * c->b.size();
+    SBDG_CHECK(hipMemcpy(hii, c->d_hii.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_sigma(sbdg_ctx* c, const double* x, double* y, int precision) {
+  return guard(c, [&]() {
+    if (!x || !y) return fail(c, SBDG_ERR_ARG, "null vector");
+    if (precision != 0 && precision != 1) return fail(c, SBDG_ERR_ARG, "precision must be 0 or 1");
+    ensure_built(c);
+    const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+    DBuf<double> in, sep, out;
+    in.alloc(N);
+    sep.alloc(N);
+    out.alloc(N);
+    SBDG_CHECK(hipMemcpyAsync(in.p, x, sizeof(double) * N, hipMemcpyHostToDevice, c->st));
+    sign_transform(c->d_a.p, na, c->d_b.p, nb, in.p, sep.p, c->st);
+    if (precision == 0) {
+      sigma64(c, sep.p, out.p);
+    } else {
+      DBuf<float> x32, y32;
+      x32.alloc(N);
+      y32.alloc(N);
+      cast_d2f(sep.p, x32.p, N, c->st);
+      sigma32(c, x32.p, y32.p);
+      cast_f2d(y32.p, out.p, N, c->st);
+      SBDG_CHECK(hipStreamSynchronize(c->st));
+    }
+    sign_transform(c->d_a.p, na, c->d_b.p, nb, out.p, in.p, c->st);
+    SBDG_CHECK(hipMemcpyAsync(y, in.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_bench_sigma(sbdg_ctx* c, int precision, int reps, double* ms) {
+  return guard(c, [&]() {
+    if (!ms || reps < 1) return fail(c, SBDG_ERR_ARG, "bad arguments");
+    ensure_built(c);
+    const int64_t N = (int64_t)c->a.size() * c->b.size();
+    hipEvent_t ev[6];
+    for (auto& e : ev) SBDG_CHECK(hipEventCreate(&e));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    std::mt19937_64 g(7);
+    std::normal_distribution<double> nd;
+    std::vector<double> h(N);
+    for (auto& x : h) x = nd(g);
+    DBuf<double> x64, y64;
+    DBuf<float> x32, y32;
+    x64.alloc(N);
+    y64.alloc(N);
+    SBDG_CHECK(hipMemcpy(x64.p, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    if (precision == 1) {
+      x32.alloc(N);
+      y32.alloc(N);
+      cast_d2f(x64.p, x32.p, N, c->st);
+    }
+    for (int r = 0; r < reps; ++r) {
+      if (precision == 1) sigma32(c, x32.p, y32.p, ev);
+      else sigma64(c, x64.p, y64.p, ev);
+      SBDG_CHECK(hipEventSynchronize(ev[5]));
+      float t;
+      SBDG_CHECK(hipEventElapsedTime(&t, ev[0], ev[5]));
+      acc[0] += t;
+      for (int k = 1; k < 6; ++k) {
+        SBDG_CHECK(hipEventElapsedTime(&t, ev[k - 1], ev[k]));
+        acc[k] += t;
+      }
+    }
+    for (int k = 0; k < 6; ++k) ms[k] = acc[k] / reps;
+    for (auto& e : ev) SBDG_CHECK(hipEventDestroy(e));
+    return SBDG_OK;
+  });
+}
+
+}  // extern "C"

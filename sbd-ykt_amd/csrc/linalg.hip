@@ -1,0 +1,421 @@
+// Vector kernels of the device-resident Davidson (tpb/davidson.h:661-924 with the
+// BLAS-1 of framework/dm_vector.h:20-84) and small helpers.
+//
+// Reductions are deterministic: each block reduces a fixed strided set of elements in a
+// fixed tree order into a partial, and one block sums the partials in index order.
+// Basis vectors live in one allocation with a leading dimension, V_k = V + k*ld.
+#include <algorithm>
+
+#include "sbdg_internal.h"
+
+namespace sbdg {
+
+namespace {
+
+constexpr int RT = 256;   // reduction block
+constexpr int RB = 1024;  // reduction blocks (upper bound)
+
+inline unsigned nblocks(int64_t n) {
+  int64_t b = (n + RT * 4 - 1) / (RT * 4);
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(RB, b));
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block tree sum of v (all threads receive it); sh has RT/64 doubles
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+  return t;
+}
+
+// partial[blk*m + k] -> out[k], one block, fixed order
+__global__ void __launch_bounds__(RT) k_final(const double* __restrict__ part, int nblk, int m,
+                                              double* __restrict__ out) {
+  __shared__ double sh[RT / 64];
+  for (int k = 0; k < m; ++k) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += RT) v += part[(int64_t)b * m + k];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) out[k] = v;
+  }
+}
+
+__global__ void k_sign(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, int64_t nb,
+                       int64_t N, const double* __restrict__ in, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const int64_t ia = t / nb, ib = t - ia * nb;
+  uint64_t A = a[ia];
+  const uint64_t B = b[ib];
+  int par = 0;
+  while (A) {
+    const int p = __builtin_ctzll(A);
+    A &= A - 1;
+    par += __popcll(B & ((1ull << p) - 1ull));
+  }
+  out[t] = (par & 1) ? -in[t] : in[t];
+}
+
+__global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    out[t] = (float)in[t];
+}
+__global__ void k_f2d(const float* __restrict__ in, double* __restrict__ out, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    out[t] = (double)in[t];
+}
+
+template <class TY, int MM>
+__global__ void __launch_bounds__(RT) k_mdot(const double* __restrict__ X, int64_t ldx, int m,
+                                             const TY* __restrict__ y, int64_t n,
+                                             double* __restrict__ part) {
+  __shared__ double sh[RT / 64];
+  double acc[MM];
+#pragma unroll
+  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+    const double yi = (double)y[i];
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m) acc[k] += X[k * ldx + i] * yi;
+  }
+#pragma unroll
+  for (int k = 0; k < MM; ++k) {
+    if (k < m) {
+      double v = block_sum(acc[k], sh);
+      if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m + k] = v;
+    }
+  }
+}
+
+template <class TH, int MM>
+__global__ void __launch_bounds__(RT) k_ritz(const double* __restrict__ V, int64_t ldv,
+                                             const TH* __restrict__ HV, int64_t ldh, int m,
+                                             const double* __restrict__ u,
+                                             const double* __restrict__ theta,
+                                             double* __restrict__ W, double* __restrict__ R, int64_t n,
+                                             double* __restrict__ part) {
+  __shared__ double sh[RT / 64];
+  double uk[MM];
+#pragma unroll
+  for (int k = 0; k < MM; ++k) uk[k] = (k < m) ? u[k] : 0.0;
+  const double th = theta[0];
+  double sw = 0.0, sr = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+    double w = 0.0, h = 0.0;
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m) {
+        w += uk[k] * V[k * ldv + i];
+        h += uk[k] * (double)HV[k * ldh + i];
+      }
+    const double r = h - th * w;
+    W[i] = w;
+    R[i] = r;
+    sw += w * w;
+    sr += r * r;
+  }
+  sw = block_sum(sw, sh);
+  sr = block_sum(sr, sh);
+  if (threadIdx.x == 0) {
+    part[(int64_t)blockIdx.x * 2] = sw;
+    part[(int64_t)blockIdx.x * 2 + 1] = sr;
+  }
+}
+
+__global__ void k_scale(double* __restrict__ x, const double* __restrict__ n2, int64_t n) {
+  const double f = 1.0 / sqrt(n2[0]);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    x[t] *= f;
+}
+
+__global__ void k_precond(const double* __restrict__ R, const double* __restrict__ rn2,
+                          const double* __restrict__ hii, const double* __restrict__ theta,
+                          double* __restrict__ t, int64_t n) {
+  const double f = 1.0 / sqrt(rn2[0]);
+  const double th = theta[0];
+  const double eps_reg = 1.0e-12;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double den = th - hii[i];
+    const double r = R[i] * f;
+    t[i] = (fabs(den) > eps_reg) ? r / den : r / (den - eps_reg);
+  }
+}
+
+template <int MM>
+__global__ void __launch_bounds__(RT) k_gs(const double* __restrict__ V, int64_t ldv, int m,
+                                           const double* __restrict__ s, double* __restrict__ t, int64_t n,
+                                           double* __restrict__ part) {
+  __shared__ double sh[RT / 64];
+  double sk[MM];
+#pragma unroll
+  for (int k = 0; k < MM; ++k) sk[k] = (k < m) ? s[k] : 0.0;
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+    double v = t[i];
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m) v -= sk[k] * V[k * ldv + i];
+    t[i] = v;
+    acc += v * v;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+template <class T, int MM>
+__global__ void k_rotate(T* __restrict__ X, int64_t ldx, int m, const double* __restrict__ U, int ld, int mk,
+                         int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double x[MM];
+#pragma unroll
+    for (int k = 0; k < MM; ++k) x[k] = (k < m) ? (double)X[k * ldx + i] : 0.0;
+    for (int j = 0; j < mk; ++j) {
+      double y = 0.0;
+#pragma unroll
+      for (int k = 0; k < MM; ++k)
+        if (k < m) y += U[k + ld * j] * x[k];
+      X[j * ldx + i] = (T)y;
+    }
+  }
+}
+
+// cyclic Jacobi on one wave; A, V in LDS (m <= 64)
+__global__ void __launch_bounds__(64) k_jacobi(const double* __restrict__ H, int m, int ld,
+                                               double* __restrict__ E, double* __restrict__ U) {
+  __shared__ double A[64 * 64];
+  __shared__ double Vv[64 * 64];
+  __shared__ int ord[64];
+  const int k = threadIdx.x;
+  for (int j = 0; j < m; ++j)
+    if (k < m) {
+      const int r = k <= j ? k : j, c = k <= j ? j : k;  // upper triangle is authoritative
+      A[k + 64 * j] = H[r + ld * c];
+      Vv[k + 64 * j] = (k == j) ? 1.0 : 0.0;
+    }
+  __syncthreads();
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    if (k < m)
+      for (int j = 0; j < m; ++j) {
+        const double v = A[k + 64 * j];
+        tot += v * v;
+        if (j != k) off += v * v;
+      }
+    off = wave_sum(off);
+    tot = wave_sum(tot);
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < m - 1; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = A[p + 64 * q];
+        if (fabs(apq) < 1e-300) continue;
+        const double app = A[p + 64 * p], aqq = A[q + 64 * q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        __syncthreads();
+        if (k < m) {
+          const double akp = A[k + 64 * p], akq = A[k + 64 * q];
+          A[k + 64 * p] = c * akp - s * akq;
+          A[k + 64 * q] = s * akp + c * akq;
+        }
+        __syncthreads();
+        if (k < m) {
+          const double apk = A[p + 64 * k], aqk = A[q + 64 * k];
+          A[p + 64 * k] = c * apk - s * aqk;
+          A[q + 64 * k] = s * apk + c * aqk;
+          const double vkp = Vv[k + 64 * p], vkq = Vv[k + 64 * q];
+          Vv[k + 64 * p] = c * vkp - s * vkq;
+          Vv[k + 64 * q] = s * vkp + c * vkq;
+        }
+        __syncthreads();
+      }
+  }
+  if (k == 0) {
+    for (int i = 0; i < m; ++i) ord[i] = i;
+    for (int i = 0; i < m; ++i) {
+      int mi = i;
+      for (int j = i + 1; j < m; ++j)
+        if (A[ord[j] * 65] < A[ord[mi] * 65]) mi = j;
+      const int t = ord[i];
+      ord[i] = ord[mi];
+      ord[mi] = t;
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {
+    if (k == 0) E[j] = A[ord[j] * 65];
+    if (k < m) U[k + ld * j] = Vv[k + 64 * ord[j]];
+  }
+}
+
+// row sums of W^2 and the per-orbital occupations -> density[2i+s]
+__global__ void __launch_bounds__(RT) k_rowsq(const double* __restrict__ W, int64_t nb, double* __restrict__ rs) {
+  __shared__ double sh[RT / 64];
+  const int64_t ia = blockIdx.x;
+  double v = 0.0;
+  for (int64_t ib = threadIdx.x; ib < nb; ib += RT) {
+    const double w = W[ia * nb + ib];
+    v += w * w;
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) rs[ia] = v;
+}
+
+__global__ void k_colsq(const double* __restrict__ W, int64_t na, int64_t nb, double* __restrict__ cs) {
+  const int64_t ib = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib >= nb) return;
+  double v = 0.0;
+  for (int64_t ia = 0; ia < na; ++ia) {
+    const double w = W[ia * nb + ib];
+    v += w * w;
+  }
+  cs[ib] = v;
+}
+
+__global__ void __launch_bounds__(RT) k_occ(const double* __restrict__ rs, const uint64_t* __restrict__ a, int64_t na,
+                                            const double* __restrict__ cs, const uint64_t* __restrict__ b,
+                                            int64_t nb, int L, double* __restrict__ dens) {
+  __shared__ double sh[RT / 64];
+  for (int o = 0; o < L; ++o) {
+    double va = 0.0, vb = 0.0;
+    for (int64_t i = threadIdx.x; i < na; i += RT)
+      if ((a[i] >> o) & 1ull) va += rs[i];
+    for (int64_t i = threadIdx.x; i < nb; i += RT)
+      if ((b[i] >> o) & 1ull) vb += cs[i];
+    va = block_sum(va, sh);
+    vb = block_sum(vb, sh);
+    if (threadIdx.x == 0) {
+      dens[2 * o] = va;
+      dens[2 * o + 1] = vb;
+    }
+  }
+}
+
+template <int MM, class F>
+void dispatch_m(int m, F&& f) {
+  if (m <= 8) f(std::integral_constant<int, 8>{});
+  else if (m <= 16) f(std::integral_constant<int, 16>{});
+  else if (m <= 32) f(std::integral_constant<int, 32>{});
+  else if (m <= 64) f(std::integral_constant<int, 64>{});
+  else throw Error(SBDG_ERR_LIMIT, "more than 64 subspace vectors");
+}
+
+}  // namespace
+
+void sign_transform(const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, const double* in, double* out,
+                    hipStream_t st) {
+  const int64_t N = na * nb;
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_sign, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, a, b, nb, N, in, out);
+  SBDG_CHECK(hipGetLastError());
+}
+
+void cast_d2f(const double* in, float* out, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_d2f, dim3(nblocks(n) * 4), dim3(256), 0, st, in, out, n);
+}
+void cast_f2d(const float* in, double* out, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_f2d, dim3(nblocks(n) * 4), dim3(256), 0, st, in, out, n);
+}
+
+void multi_dot_impl(const double* X, int64_t ldx, int m, const double* y, int64_t n, double* out, double* part,
+                    hipStream_t st) {
+  const unsigned g = nblocks(n);
+  dispatch_m<0>(m, [&](auto MM) {
+    hipLaunchKernelGGL((k_mdot<double, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+  });
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, m, out);
+  SBDG_CHECK(hipGetLastError());
+}
+
+void multi_dot_impl(const double* X, int64_t ldx, int m, const float* y, int64_t n, double* out, double* part,
+                    hipStream_t st) {
+  const unsigned g = nblocks(n);
+  dispatch_m<0>(m, [&](auto MM) {
+    hipLaunchKernelGGL((k_mdot<float, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+  });
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, m, out);
+  SBDG_CHECK(hipGetLastError());
+}
+
+template <class TH>
+void ritz_combine_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, int m, const double* u,
+                       const double* theta, double* W, double* R, int64_t n, double* out, double* part,
+                       hipStream_t st) {
+  const unsigned g = nblocks(n);
+  dispatch_m<0>(m, [&](auto MM) {
+    hipLaunchKernelGGL((k_ritz<TH, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u, theta,
+                       W, R, n, part);
+  });
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, 2, out);
+  SBDG_CHECK(hipGetLastError());
+}
+template void ritz_combine_impl<double>(const double*, int64_t, const double*, int64_t, int, const double*,
+                                        const double*, double*, double*, int64_t, double*, double*, hipStream_t);
+template void ritz_combine_impl<float>(const double*, int64_t, const float*, int64_t, int, const double*,
+                                       const double*, double*, double*, int64_t, double*, double*, hipStream_t);
+
+void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_scale, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
+}
+
+void precond_impl(const double* R, const double* rn2, const double* hii, const double* theta, double* t, int64_t n,
+                  hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_precond, dim3(nblocks(n) * 4), dim3(256), 0, st, R, rn2, hii, theta, t, n);
+}
+
+void gs_impl(const double* V, int64_t ldv, int m, const double* s, double* t, int64_t n, double* out, double* part,
+             hipStream_t st) {
+  const unsigned g = nblocks(n);
+  dispatch_m<0>(m, [&](auto MM) {
+    hipLaunchKernelGGL((k_gs<decltype(MM)::value>), dim3(g), dim3(RT), 0, st, V, ldv, m, s, t, n, part);
+  });
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, 1, out);
+  SBDG_CHECK(hipGetLastError());
+}
+
+template <class T>
+void rotate_impl(T* X, int64_t ldx, int m, const double* U, int ld, int mk, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  dispatch_m<0>(m, [&](auto MM) {
+    hipLaunchKernelGGL((k_rotate<T, decltype(MM)::value>), dim3(nblocks(n) * 4), dim3(256), 0, st, X, ldx, m, U, ld,
+                       mk, n);
+  });
+  SBDG_CHECK(hipGetLastError());
+}
+template void rotate_impl<double>(double*, int64_t, int, const double*, int, int, int64_t, hipStream_t);
+template void rotate_impl<float>(float*, int64_t, int, const double*, int, int, int64_t, hipStream_t);
+
+void syev_device(const double* d_H, int m, int ld, double* d_E, double* d_U, hipStream_t st) {
+  if (m > 64) throw Error(SBDG_ERR_LIMIT, "subspace beyond 64");
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(64), 0, st, d_H, m, ld, d_E, d_U);
+  SBDG_CHECK(hipGetLastError());
+}
+
+void density(const double* W, const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, int L, double* d_dens,
+             double* d_part, hipStream_t st) {
+  double* rs = d_part;
+  double* cs = d_part + na;
+  if (na > 0) hipLaunchKernelGGL(k_rowsq, dim3((unsigned)na), dim3(RT), 0, st, W, nb, rs);
+  if (nb > 0) hipLaunchKernelGGL(k_colsq, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, W, na, nb, cs);
+  hipLaunchKernelGGL(k_occ, dim3(1), dim3(RT), 0, st, rs, a, na, cs, b, nb, L, d_dens);
+  SBDG_CHECK(hipGetLastError());
+}
+
+}  // namespace sbdg

@@ -1,0 +1,545 @@
+// 1- and 2-particle RDMs of a real vector (replaces Correlation, tpb/correlation.h:194-444,
+// and feeds the Hellmann-Feynman backward).
+//
+// Output layout is the reference's: one[s][p + L q] = <c+_{p s} c_{q s}>,
+// two[s + 2t][i + L j + L^2 k + L^3 l] = <c+_{i s} c+_{j t} c_{l t} c_{k s}>.
+// Mixed-spin blocks are the physical ones.
+//
+// Work is organised by which string pairs connect bra and ket (separated phase order,
+// W real, tables symmetric so both orders of every pair are visited):
+//  * alpha pairs (ia,ja), beta fixed: g = sum_ib W[ia,ib] W[ja,ib]; singles also need the
+//    beta-occupation-resolved sums gb[k] = sum_ib W[ia,ib] W[ja,ib] n_k(ib).  Beta pairs
+//    are the same on the transposed vector.  (row-pair Gram products, one workgroup per
+//    bra row, LDS accumulators)
+//  * alpha-beta pairs: T_(a,i)[b,j] = sum sa sb W[ia,ib] W[ja,jb] over alpha singles with
+//    orbitals (a<-i) and beta singles (b<-j); one workgroup per group of alpha pairs with
+//    one key (a,i), the ket row staged in LDS, accumulators in LDS.
+//  * the diagonal: density correlators sum_D W_D^2 n_p n_q.
+// Every RDM slot is written by exactly one owner (singles by their key (a,i), doubles by
+// their orbital quadruple, alpha-beta by (a,i), the diagonal by (p,q)): no global atomics.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+#include "sbdg_internal.h"
+
+namespace sbdg {
+
+void transpose_d(int64_t R, int64_t C, const double* in, double* out, hipStream_t st);
+
+namespace {
+
+constexpr int RT = 256;
+
+__device__ inline double block_sum_d(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+  return t;
+}
+
+// g[q] = <X[r,:], X[col[q],:]>; OCC: gocc[q*L+k] = sum_c X[r,c] X[col[q],c] n_k(cstr[c])
+template <bool OCC>
+__global__ void __launch_bounds__(RT) k_gram(int64_t ncols, int L, const int64_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ col, const double* __restrict__ X,
+                                             const uint64_t* __restrict__ cstr, double* __restrict__ g,
+                                             double* __restrict__ gocc) {
+  __shared__ double sh[RT / 64];
+  __shared__ double acc[64];
+  const int64_t r = blockIdx.x;
+  const double* xr = X + r * ncols;
+  for (int64_t q = rowptr[r]; q < rowptr[r + 1]; ++q) {
+    const double* xq = X + (int64_t)col[q] * ncols;
+    if (OCC) {
+      if (threadIdx.x < 64) acc[threadIdx.x] = 0.0;
+      __syncthreads();
+    }
+    double s = 0.0;
+    for (int64_t cc = threadIdx.x; cc < ncols; cc += RT) {
+      const double p = xr[cc] * xq[cc];
+      s += p;
+      if (OCC && p != 0.0) {
+        uint64_t y = cstr[cc];
+        while (y) {
+          const int k = __builtin_ctzll(y);
+          y &= y - 1;
+          atomicAdd(&acc[k], p);
+        }
+      }
+    }
+    s = block_sum_d(s, sh);
+    if (threadIdx.x == 0) g[q] = s;
+    if (OCC) {
+      __syncthreads();
+      if (threadIdx.x < L) gocc[q * L + threadIdx.x] = acc[threadIdx.x];
+      __syncthreads();
+    }
+  }
+}
+
+// per row: rsq[r] = sum_c X^2, P[r*L+k] = sum_c X[r,c]^2 n_k(cstr[c])
+__global__ void __launch_bounds__(RT) k_rowocc(int64_t ncols, int L, const double* __restrict__ X,
+                                               const uint64_t* __restrict__ cstr, double* __restrict__ rsq,
+                                               double* __restrict__ P) {
+  __shared__ double sh[RT / 64];
+  __shared__ double acc[64];
+  const int64_t r = blockIdx.x;
+  if (threadIdx.x < 64) acc[threadIdx.x] = 0.0;
+  __syncthreads();
+  double s = 0.0;
+  for (int64_t cc = threadIdx.x; cc < ncols; cc += RT) {
+    const double x = X[r * ncols + cc];
+    const double p = x * x;
+    s += p;
+    if (p != 0.0) {
+      uint64_t y = cstr[cc];
+      while (y) {
+        const int k = __builtin_ctzll(y);
+        y &= y - 1;
+        atomicAdd(&acc[k], p);
+      }
+    }
+  }
+  s = block_sum_d(s, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) rsq[r] = s;
+  if (threadIdx.x < L) P[r * L + threadIdx.x] = acc[threadIdx.x];
+}
+
+// D[i][j] = sum_r w[r] n_i(rstr[r]) * (Y ? Y[r*L+j] : n_j(rstr[r]))  (one thread per (i,j))
+__global__ void k_corr(int64_t nrows, int L, const uint64_t* __restrict__ rstr, const double* __restrict__ w,
+                       const double* __restrict__ Y, double* __restrict__ D) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L * L) return;
+  const int i = t / L, j = t % L;
+  double v = 0.0;
+  for (int64_t r = 0; r < nrows; ++r) {
+    const uint64_t s = rstr[r];
+    if (!((s >> i) & 1ull)) continue;
+    if (Y) v += Y[r * L + j];
+    else if ((s >> j) & 1ull) v += w[r];
+  }
+  D[t] = v;
+}
+
+__device__ inline int64_t idx4(int L, int i, int j, int k, int l) {
+  return (int64_t)i + (int64_t)L * j + (int64_t)L * L * k + (int64_t)L * L * L * l;
+}
+
+// diagonal slots: gamma_s[p,p], Gamma_st[i,j,i,j], Gamma_ss[i,j,j,i]
+__global__ void k_diag_slots(int L, const double* __restrict__ dens, const double* __restrict__ Daa,
+                             const double* __restrict__ Dbb, const double* __restrict__ Dab,
+                             double* __restrict__ one, double* __restrict__ two) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L * L) return;
+  const int i = t / L, j = t % L;
+  const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
+  if (i == j) {
+    one[i + L * i] = dens[2 * i];
+    one[L2 + i + L * i] = dens[2 * i + 1];
+  }
+  // alpha-beta (block 2) and beta-alpha (block 1): every (i,j)
+  two[2 * L4 + idx4(L, i, j, i, j)] = Dab[i * L + j];
+  two[1 * L4 + idx4(L, j, i, j, i)] = Dab[i * L + j];
+  if (i != j) {
+    two[0 * L4 + idx4(L, i, j, i, j)] = Daa[i * L + j];
+    two[0 * L4 + idx4(L, i, j, j, i)] = -Daa[i * L + j];
+    two[3 * L4 + idx4(L, i, j, i, j)] = Dbb[i * L + j];
+    two[3 * L4 + idx4(L, i, j, j, i)] = -Dbb[i * L + j];
+  }
+}
+
+// singles owners: one thread per key (a,i); its pairs are perm[seg[key]..seg[key+1]).
+// spin s (0 alpha / 1 beta); spectators of the same spin from the ket string, of the
+// other spin from gocc.  blk(s,t) = s + 2t.
+__global__ void k_single_slots(int L, int s, const int32_t* __restrict__ seg, const int32_t* __restrict__ perm,
+                               const int64_t* __restrict__ rowof, const int32_t* __restrict__ col,
+                               const uint32_t* __restrict__ exc, const uint64_t* __restrict__ str,
+                               const double* __restrict__ g, const double* __restrict__ gocc,
+                               double* __restrict__ one, double* __restrict__ two) {
+  const int key = blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= L * L) return;
+  const int a = key / L, i = key % L;
+  if (a == i) return;
+  const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
+  double accS[64], accO[64];
+  for (int k = 0; k < L; ++k) accS[k] = accO[k] = 0.0;
+  double gam = 0.0;
+  for (int32_t e = seg[key]; e < seg[key + 1]; ++e) {
+    const int32_t q = perm[e];
+    const double sg = ((exc[q] >> 24) & 1u) ? -1.0 : 1.0;
+    const double v = sg * g[q];
+    gam += v;
+    uint64_t sp = str[col[q]] & ~(1ull << i);  // ket occupation minus i = common occupation
+    while (sp) {
+      const int k = __builtin_ctzll(sp);
+      sp &= sp - 1;
+      accS[k] += v;
+    }
+    for (int k = 0; k < L; ++k) accO[k] += sg * gocc[(int64_t)q * L + k];
+  }
+  (void)rowof;
+  const int ss = s + 2 * s;          // same-spin block
+  const int so = s + 2 * (1 - s);    // (s, other)
+  const int os = (1 - s) + 2 * s;    // (other, s)
+  one[s * L2 + a + L * i] = gam;
+  for (int k = 0; k < L; ++k) {
+    if (k != a && k != i) {
+      const double v = accS[k];
+      two[ss * L4 + idx4(L, a, k, i, k)] = v;
+      two[ss * L4 + idx4(L, k, a, k, i)] = v;
+      two[ss * L4 + idx4(L, a, k, k, i)] = -v;
+      two[ss * L4 + idx4(L, k, a, i, k)] = -v;
+    }
+    const double w = accO[k];
+    two[so * L4 + idx4(L, a, k, i, k)] = w;
+    two[os * L4 + idx4(L, k, a, k, i)] = w;
+  }
+}
+
+// doubles owners: one thread per distinct quadruple (segment of sorted keys)
+__global__ void k_double_slots(int L, int s, int64_t nseg, const int32_t* __restrict__ seg,
+                               const int32_t* __restrict__ perm, const uint32_t* __restrict__ keys,
+                               const uint32_t* __restrict__ exc, const double* __restrict__ g,
+                               double* __restrict__ two) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nseg) return;
+  const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
+  double v = 0.0;
+  for (int32_t e = seg[t]; e < seg[t + 1]; ++e) {
+    const int32_t q = perm[e];
+    v += (((exc[q] >> 24) & 1u) ? -1.0 : 1.0) * g[q];
+  }
+  const uint32_t key = keys[seg[t]];
+  const int J = key % L, I = (key / L) % L, B = (key / L / L) % L, A = key / L / L / L;
+  const int ss = s + 2 * s;
+  two[ss * L4 + idx4(L, A, B, I, J)] = v;
+  two[ss * L4 + idx4(L, B, A, J, I)] = v;
+  two[ss * L4 + idx4(L, A, B, J, I)] = -v;
+  two[ss * L4 + idx4(L, B, A, I, J)] = -v;
+}
+
+__global__ void k_single_keys(int64_t nnz, int L, const uint32_t* __restrict__ exc, uint32_t* __restrict__ keys,
+                              int32_t* __restrict__ idx) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  const uint32_t e = exc[q];
+  keys[q] = ((e >> 6) & 63u) * (uint32_t)L + (e & 63u);  // a*L + i
+  idx[q] = (int32_t)q;
+}
+
+__global__ void k_double_keys(int64_t nnz, int L, const uint32_t* __restrict__ exc, uint32_t* __restrict__ keys,
+                              int32_t* __restrict__ idx) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  const uint32_t e = exc[q];
+  const uint32_t I = e & 63u, J = (e >> 6) & 63u, A = (e >> 12) & 63u, B = (e >> 18) & 63u;
+  keys[q] = ((A * L + B) * L + I) * L + J;
+  idx[q] = (int32_t)q;
+}
+
+// segment starts of sorted keys: flag[e] = (e==0 || key[e] != key[e-1])
+__global__ void k_flag(int64_t n, const uint32_t* __restrict__ keys, int32_t* __restrict__ flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  flag[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1 : 0;
+}
+
+__global__ void k_segstart(int64_t n, const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                           int32_t* __restrict__ seg, int32_t nseg) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n && flag[e]) seg[pos[e]] = (int32_t)e;
+  if (e == 0) seg[nseg] = (int32_t)n;
+}
+
+// per key (a,i) dense segment table: kseg[key] = first sorted entry with key >= key
+__global__ void k_keyseg(int64_t n, int nkeys, const uint32_t* __restrict__ keys, int32_t* __restrict__ kseg) {
+  const int key = blockIdx.x * blockDim.x + threadIdx.x;
+  if (key > nkeys) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int)keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  kseg[key] = (int32_t)lo;
+}
+
+// alpha-beta: workgroup per (key, chunk of <= PCH pairs); partial[T][L*L] in
+// part[(kseg_chunk)*L2].  LDS: ket row chunk + L*L accumulators.
+constexpr int PCH = 32;
+__global__ void __launch_bounds__(RT) k_ab_rdm(int64_t nb, int L, const int32_t* __restrict__ wseg,
+                                               const int32_t* __restrict__ wkey, const int32_t* __restrict__ perm,
+                                               const int64_t* __restrict__ rowof, const int32_t* __restrict__ colSa,
+                                               const uint32_t* __restrict__ excSa,
+                                               const uint32_t* __restrict__ pack,
+                                               const int32_t* __restrict__ cptr, int nch, int64_t chunk,
+                                               const double* __restrict__ W, double* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* cb = reinterpret_cast<double*>(smem);
+  double* T = cb + chunk;
+  const int LL = L * L;
+  const int w = blockIdx.x;
+  for (int x = threadIdx.x; x < LL; x += RT) T[x] = 0.0;
+  for (int32_t e = wseg[w]; e < wseg[w + 1]; ++e) {
+    const int32_t q = perm[e];
+    const int64_t ia = rowof[q], ja = colSa[q];
+    const double sa = ((excSa[q] >> 24) & 1u) ? -1.0 : 1.0;
+    const double* wi = W + ia * nb;
+    for (int k = 0; k < nch; ++k) {
+      const int64_t j0 = (int64_t)k * chunk, j1 = min(nb, j0 + chunk);
+      __syncthreads();
+      for (int64_t x = threadIdx.x; x < j1 - j0; x += RT) cb[x] = W[ja * nb + j0 + x];
+      __syncthreads();
+      for (int64_t ib = threadIdx.x; ib < nb; ib += RT) {
+        const double wb = sa * wi[ib];
+        if (wb == 0.0) continue;
+        const int32_t* cp = cptr + ib * (nch + 1) + k;
+        for (int ee = cp[0]; ee < cp[1]; ++ee) {
+          const uint32_t p = pack[ee];
+          const int jb = (int)(p & 0x3FFFFu) - (int)j0;
+          const double v = (((p >> 30) & 1u) ? -wb : wb) * cb[jb];
+          atomicAdd(&T[(p >> 18) & 0xFFFu], v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  (void)wkey;
+  for (int x = threadIdx.x; x < LL; x += RT) part[(int64_t)w * LL + x] = T[x];
+}
+
+// sum the chunk partials of each key (a,i) and write Gamma_ab[a,b,i,j], Gamma_ba[b,a,j,i]
+__global__ void k_ab_slots(int L, const int32_t* __restrict__ kfirst, const double* __restrict__ part,
+                           double* __restrict__ two) {
+  const int key = blockIdx.x;
+  const int a = key / L, i = key % L;
+  const int LL = L * L;
+  const int64_t L4 = (int64_t)LL * LL;
+  for (int x = threadIdx.x; x < LL; x += blockDim.x) {
+    double v = 0.0;
+    for (int32_t c = kfirst[key]; c < kfirst[key + 1]; ++c) v += part[(int64_t)c * LL + x];
+    const int b = x / L, j = x % L;
+    if (a == i || b == j) continue;
+    two[2 * L4 + idx4(L, a, b, i, j)] = v;
+    two[1 * L4 + idx4(L, b, a, j, i)] = v;
+  }
+}
+
+__global__ void k_rowof(int64_t n, const int64_t* __restrict__ rowptr, int64_t* __restrict__ rowof) {
+  const int64_t r = blockIdx.x;
+  for (int64_t q = rowptr[r] + threadIdx.x; q < rowptr[r + 1]; q += blockDim.x) rowof[q] = r;
+  (void)n;
+}
+
+__global__ void k_backward(int L, const double* __restrict__ one, const double* __restrict__ two, double g,
+                           double* __restrict__ dh, double* __restrict__ deri) {
+  const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < L4 && deri) {
+    const int s_ = (int)(t % L), r = (int)((t / L) % L), q = (int)((t / L2) % L), p = (int)(t / L2 / L);
+    const int64_t x = idx4(L, p, r, q, s_);
+    deri[t] = 0.5 * g * (two[x] + two[L4 + x] + two[2 * L4 + x] + two[3 * L4 + x]);
+  }
+  if (t < L2 && dh) {
+    const int q = (int)(t % L), p = (int)(t / L);
+    dh[t] = g * (one[p + L * q] + one[L2 + p + L * q]);
+  }
+}
+
+inline unsigned nb_for(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+// sort (key, index) pairs; returns sorted keys/indices in the out buffers
+void sort_pairs(int64_t n, DBuf<uint32_t>& keys, DBuf<int32_t>& idx, DBuf<uint32_t>& keys_out,
+                DBuf<int32_t>& idx_out, int end_bit, hipStream_t st) {
+  keys_out.alloc(std::max<int64_t>(n, 1));
+  idx_out.alloc(std::max<int64_t>(n, 1));
+  if (n == 0) return;
+  size_t tmp = 0;
+  SBDG_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys.p, keys_out.p, idx.p, idx_out.p, (int)n, 0,
+                                                end_bit, st));
+  DBuf<unsigned char> t;
+  t.alloc(tmp + 1);
+  SBDG_CHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys.p, keys_out.p, idx.p, idx_out.p, (int)n, 0, end_bit,
+                                                st));
+  SBDG_CHECK(hipStreamSynchronize(st));
+}
+
+int bits_for(uint64_t v) {
+  int b = 1;
+  while ((1ull << b) <= v) ++b;
+  return b;
+}
+
+struct PairBlock {  // g (and gocc for singles) of one table
+  DBuf<double> g, gocc;
+};
+
+// singles of one spin: Gram products, then key owners
+void singles_part(const Table& t, int L, int s, const double* X, int64_t ncols, const uint64_t* rstr,
+                  const uint64_t* cstr, double* one, double* two, hipStream_t st) {
+  if (t.nnz == 0) return;
+  PairBlock pb;
+  pb.g.alloc(t.nnz);
+  pb.gocc.alloc((size_t)t.nnz * L);
+  hipLaunchKernelGGL(k_gram<true>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, L, t.rowptr.p, t.col.p, X, cstr,
+                     pb.g.p, pb.gocc.p);
+  DBuf<uint32_t> keys, ks;
+  DBuf<int32_t> idx, perm, kseg;
+  keys.alloc(t.nnz);
+  idx.alloc(t.nnz);
+  hipLaunchKernelGGL(k_single_keys, dim3(nb_for(t.nnz, 256)), dim3(256), 0, st, t.nnz, L, t.exc.p, keys.p, idx.p);
+  sort_pairs(t.nnz, keys, idx, ks, perm, bits_for((uint64_t)L * L), st);
+  kseg.alloc(L * L + 1);
+  hipLaunchKernelGGL(k_keyseg, dim3(nb_for(L * L + 1, 256)), dim3(256), 0, st, t.nnz, L * L, ks.p, kseg.p);
+  hipLaunchKernelGGL(k_single_slots, dim3(nb_for(L * L, 64)), dim3(64), 0, st, L, s, kseg.p, perm.p, nullptr,
+                     t.col.p, t.exc.p, rstr, pb.g.p, pb.gocc.p, one, two);
+  SBDG_CHECK(hipGetLastError());
+  SBDG_CHECK(hipStreamSynchronize(st));
+}
+
+void doubles_part(const Table& t, int L, int s, const double* X, int64_t ncols, double* two, hipStream_t st) {
+  if (t.nnz == 0) return;
+  DBuf<double> g;
+  g.alloc(t.nnz);
+  hipLaunchKernelGGL(k_gram<false>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, L, t.rowptr.p, t.col.p, X, nullptr,
+                     g.p, nullptr);
+  DBuf<uint32_t> keys, ks;
+  DBuf<int32_t> idx, perm, flag, pos, seg;
+  keys.alloc(t.nnz);
+  idx.alloc(t.nnz);
+  hipLaunchKernelGGL(k_double_keys, dim3(nb_for(t.nnz, 256)), dim3(256), 0, st, t.nnz, L, t.exc.p, keys.p, idx.p);
+  sort_pairs(t.nnz, keys, idx, ks, perm, bits_for((uint64_t)L * L * L * L), st);
+  flag.alloc(t.nnz);
+  pos.alloc(t.nnz);
+  hipLaunchKernelGGL(k_flag, dim3(nb_for(t.nnz, 256)), dim3(256), 0, st, t.nnz, ks.p, flag.p);
+  size_t tmp = 0;
+  SBDG_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flag.p, pos.p, (int)t.nnz, st));
+  DBuf<unsigned char> tb;
+  tb.alloc(tmp + 1);
+  SBDG_CHECK(hipcub::DeviceScan::ExclusiveSum(tb.p, tmp, flag.p, pos.p, (int)t.nnz, st));
+  int32_t lastpos = 0, lastflag = 0;
+  SBDG_CHECK(hipMemcpyAsync(&lastpos, pos.p + t.nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  SBDG_CHECK(hipMemcpyAsync(&lastflag, flag.p + t.nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  SBDG_CHECK(hipStreamSynchronize(st));
+  const int32_t nseg = lastpos + lastflag;
+  seg.alloc(nseg + 1);
+  hipLaunchKernelGGL(k_segstart, dim3(nb_for(t.nnz, 256)), dim3(256), 0, st, t.nnz, flag.p, pos.p, seg.p, nseg);
+  hipLaunchKernelGGL(k_double_slots, dim3(nb_for(nseg, 256)), dim3(256), 0, st, L, s, (int64_t)nseg, seg.p, perm.p,
+                     ks.p, t.exc.p, g.p, two);
+  SBDG_CHECK(hipGetLastError());
+  SBDG_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+void rdm_compute(const RdmPlan& P, const double* W, double* one, double* two, hipStream_t st) {
+  const int L = P.L;
+  const int64_t na = P.na, nb = P.nb, N = na * nb;
+  const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
+  SBDG_CHECK(hipMemsetAsync(one, 0, sizeof(double) * 2 * L2, st));
+  SBDG_CHECK(hipMemsetAsync(two, 0, sizeof(double) * 4 * L4, st));
+  if (N == 0) return;
+  DBuf<double> WT;
+  WT.alloc(N);
+  transpose_d(na, nb, W, WT.p, st);
+
+  // diagonal: density correlators
+  {
+    DBuf<double> rsa, Pa, rsb, Pb, Daa, Dbb, Dab, dens;
+    rsa.alloc(na);
+    Pa.alloc(na * L);
+    rsb.alloc(nb);
+    Pb.alloc(nb * L);
+    Daa.alloc(L2);
+    Dbb.alloc(L2);
+    Dab.alloc(L2);
+    dens.alloc(2 * L);
+    // rows of W: rsa[ia] = sum W^2, Pa[ia][j] = sum_ib W^2 n_j(ib)  (beta occupations)
+    hipLaunchKernelGGL(k_rowocc, dim3((unsigned)na), dim3(RT), 0, st, nb, L, W, P.b, rsa.p, Pa.p);
+    hipLaunchKernelGGL(k_rowocc, dim3((unsigned)nb), dim3(RT), 0, st, na, L, WT.p, P.a, rsb.p, Pb.p);
+    hipLaunchKernelGGL(k_corr, dim3(nb_for(L2, 64)), dim3(64), 0, st, na, L, P.a, rsa.p, nullptr, Daa.p);
+    hipLaunchKernelGGL(k_corr, dim3(nb_for(L2, 64)), dim3(64), 0, st, nb, L, P.b, rsb.p, nullptr, Dbb.p);
+    hipLaunchKernelGGL(k_corr, dim3(nb_for(L2, 64)), dim3(64), 0, st, na, L, P.a, nullptr, Pa.p, Dab.p);
+    // densities: diagonal of Daa/Dbb
+    std::vector<double> hdaa(L2), hdbb(L2), hd(2 * L);
+    SBDG_CHECK(hipMemcpyAsync(hdaa.data(), Daa.p, sizeof(double) * L2, hipMemcpyDeviceToHost, st));
+    SBDG_CHECK(hipMemcpyAsync(hdbb.data(), Dbb.p, sizeof(double) * L2, hipMemcpyDeviceToHost, st));
+    SBDG_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < L; ++i) {
+      hd[2 * i] = hdaa[i * L + i];
+      hd[2 * i + 1] = hdbb[i * L + i];
+    }
+    SBDG_CHECK(hipMemcpyAsync(dens.p, hd.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_diag_slots, dim3(nb_for(L2, 64)), dim3(64), 0, st, L, dens.p, Daa.p, Dbb.p, Dab.p, one, two);
+    SBDG_CHECK(hipGetLastError());
+    SBDG_CHECK(hipStreamSynchronize(st));
+  }
+  // alpha pairs (beta spectators from the beta strings of the columns)
+  singles_part(*P.sa, L, 0, W, nb, P.a, P.b, one, two, st);
+  doubles_part(*P.da, L, 0, W, nb, two, st);
+  // beta pairs on the transposed vector (alpha spectators from the alpha strings)
+  singles_part(*P.sb, L, 1, WT.p, na, P.b, P.a, one, two, st);
+  doubles_part(*P.db, L, 1, WT.p, na, two, st);
+
+  // alpha-beta
+  const Table& sa = *P.sa;
+  if (sa.nnz > 0 && P.sb->nnz > 0) {
+    DBuf<int64_t> rowof;
+    rowof.alloc(sa.nnz);
+    hipLaunchKernelGGL(k_rowof, dim3((unsigned)na), dim3(256), 0, st, na, sa.rowptr.p, rowof.p);
+    DBuf<uint32_t> keys, ks;
+    DBuf<int32_t> idx, perm, kseg;
+    keys.alloc(sa.nnz);
+    idx.alloc(sa.nnz);
+    hipLaunchKernelGGL(k_single_keys, dim3(nb_for(sa.nnz, 256)), dim3(256), 0, st, sa.nnz, L, sa.exc.p, keys.p,
+                       idx.p);
+    sort_pairs(sa.nnz, keys, idx, ks, perm, bits_for((uint64_t)L * L), st);
+    kseg.alloc(L2 + 1);
+    hipLaunchKernelGGL(k_keyseg, dim3(nb_for(L2 + 1, 256)), dim3(256), 0, st, sa.nnz, (int)L2, ks.p, kseg.p);
+    std::vector<int32_t> hk(L2 + 1);
+    SBDG_CHECK(hipMemcpyAsync(hk.data(), kseg.p, sizeof(int32_t) * (L2 + 1), hipMemcpyDeviceToHost, st));
+    SBDG_CHECK(hipStreamSynchronize(st));
+    // split each key's run into workgroups of <= PCH pairs
+    std::vector<int32_t> wseg, wkey, kfirst(L2 + 1);
+    for (int key = 0; key < (int)L2; ++key) {
+      kfirst[key] = (int32_t)wkey.size();
+      for (int32_t e = hk[key]; e < hk[key + 1]; e += PCH) {
+        wseg.push_back(e);
+        wkey.push_back(key);
+      }
+    }
+    kfirst[L2] = (int32_t)wkey.size();
+    wseg.push_back(hk[L2]);
+    const int nw = (int)wkey.size();
+    DBuf<int32_t> dws, dwk, dkf;
+    DBuf<double> part;
+    dws.alloc(wseg.size());
+    dwk.alloc(std::max(nw, 1));
+    dkf.alloc(L2 + 1);
+    part.alloc((size_t)std::max(nw, 1) * L2);
+    SBDG_CHECK(hipMemcpyAsync(dws.p, wseg.data(), sizeof(int32_t) * wseg.size(), hipMemcpyHostToDevice, st));
+    if (nw) SBDG_CHECK(hipMemcpyAsync(dwk.p, wkey.data(), sizeof(int32_t) * nw, hipMemcpyHostToDevice, st));
+    SBDG_CHECK(hipMemcpyAsync(dkf.p, kfirst.data(), sizeof(int32_t) * (L2 + 1), hipMemcpyHostToDevice, st));
+    const size_t lds = sizeof(double) * (size_t)(P.ab->chunk + L2);
+    if (nw)
+      hipLaunchKernelGGL(k_ab_rdm, dim3(nw), dim3(RT), lds, st, nb, L, dws.p, dwk.p, perm.p, rowof.p, sa.col.p,
+                         sa.exc.p, P.ab->pack.p, P.ab->cptr.p, P.ab->nch, P.ab->chunk, W, part.p);
+    hipLaunchKernelGGL(k_ab_slots, dim3((unsigned)L2), dim3(256), 0, st, L, dkf.p, part.p, two);
+    SBDG_CHECK(hipGetLastError());
+    SBDG_CHECK(hipStreamSynchronize(st));
+  }
+}
+
+void rdm_backward(const double* d_one, const double* d_two, int L, double g, double* d_dh, double* d_deri,
+                  hipStream_t st) {
+  const int64_t L4 = (int64_t)L * L * L * L;
+  hipLaunchKernelGGL(k_backward, dim3(nb_for(L4, 256)), dim3(256), 0, st, L, d_one, d_two, g, d_dh, d_deri);
+  SBDG_CHECK(hipGetLastError());
+}
+
+}  // namespace sbdg

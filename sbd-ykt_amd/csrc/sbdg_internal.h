@@ -1,0 +1,175 @@
+// Internal declarations of the MI355X selected-basis diagonalization library.
+//
+// Conventions used on the device (the C-ABI converts at its boundary):
+//  * alpha strings index rows, beta strings columns: a wavefunction is row-major
+//    [ia][ib], beta fastest, exactly the reference layout (tpb/qcham.h:359, mult.h:487);
+//  * determinants are phased in the separated order (all alpha creators, ascending, then
+//    all beta creators, ascending).  The reference phases them in interleaved order
+//    (alpha p -> spin-orbital 2p, beta q -> 2q+1; determinants.h:96-142).  The two differ
+//    by the diagonal sign s(A,B) = (-1)^(sum over p in A of #{q in B : q < p}), so
+//    H_ref = S H_sep S, sigma_ref(c) = S sigma_sep(S c), and every matrix element keeps
+//    its magnitude.  In the separated order an alpha-alpha element depends on the beta
+//    string only through the beta occupation (singles) or not at all (doubles).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sbdg.h"
+
+#define SBDG_CHECK(expr)                                                         \
+  do {                                                                           \
+    hipError_t _e = (expr);                                                      \
+    if (_e != hipSuccess) throw sbdg::Error(SBDG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+namespace sbdg {
+
+struct Error {
+  int code;
+  std::string msg;
+  Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// Device buffer owned by the context.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    free();
+    n = count;
+    if (count) SBDG_CHECK(hipMalloc(&p, sizeof(T) * count));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { free(); }
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+};
+
+// Excitation table of one string set, CSR with ket indices ascending per bra row
+// (membership and order of chemistry/tpb/helper.h:132-197).
+//  exc word (singles): bits 0-5 ket-only orbital i, 6-11 bra-only orbital a,
+//                      bit 24 set when the separated-order sign is negative.
+//  exc word (doubles): bits 0-5 I, 6-11 J (ket-only, I<J), 12-17 A, 18-23 B (bra-only,
+//                      A<B), bit 24 sign.
+//  val: the spin-string-only part of the element in the separated order:
+//       singles  sign * (h_ai + sum_{k in occ(ket)\{i}} [(ai|kk) - (ak|ki)])
+//       doubles  sign * ((AI|BJ) - (AJ|BI))
+struct Table {
+  int64_t n = 0, nnz = 0;
+  DBuf<int64_t> rowptr;  // n+1
+  DBuf<int32_t> col;     // nnz
+  DBuf<uint32_t> exc;    // nnz
+  DBuf<double> val;      // nnz
+  DBuf<float> val32;     // nnz
+};
+
+// Row-merged (singles then doubles) table feeding the row-streaming product.
+struct SpmmTable {
+  int64_t n = 0, nnz = 0;
+  DBuf<int64_t> rowptr;
+  DBuf<int32_t> col;
+  DBuf<double> val;
+  DBuf<float> val32;
+};
+
+// Packed beta singles for the alpha-beta kernel: bits 0-17 ket index jb, 18-29 orbital
+// pair b*64+j, bit 30 sign.  cptr[ib*(nch+1)+k] is the first entry of row ib whose ket
+// lies in column chunk k (entries ascend in jb, so chunks are contiguous runs).
+struct AbTable {
+  int nch = 0;
+  int64_t chunk = 0;
+  DBuf<uint32_t> pack;
+  DBuf<int32_t> cptr;
+};
+
+// kernels (tables.hip)
+void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, const double* d_eri,
+                 Table& t_single, Table& t_double, hipStream_t st);
+void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st);
+void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st);
+void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
+                 const double* d_h1, const double* d_eri, double* d_hii, hipStream_t st);
+
+// sigma (sigma.hip)
+struct SigmaPlan {
+  int64_t na = 0, nb = 0;
+  int L = 0;
+  const uint64_t* a = nullptr;
+  const uint64_t* b = nullptr;
+  const SpmmTable* ma = nullptr;  // alpha merged
+  const SpmmTable* mb = nullptr;  // beta merged
+  const Table* sa = nullptr;      // alpha singles (exc, col)
+  const AbTable* ab = nullptr;
+  const double* eri = nullptr;
+  const float* eri32 = nullptr;
+  const double* hii = nullptr;
+  const float* hii32 = nullptr;
+};
+// sigma = H c in the separated convention; scratch holds 2*na*nb elements of T.
+template <class T>
+void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream_t st,
+                 hipEvent_t* ev = nullptr);
+size_t ab_lds_bytes(int L, int64_t chunk, int elem);
+
+// vector kernels (linalg.hip)
+void sign_transform(const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, const double* in,
+                    double* out, hipStream_t st);
+void cast_d2f(const double* in, float* out, int64_t n, hipStream_t st);
+void cast_f2d(const float* in, double* out, int64_t n, hipStream_t st);
+// out[k] = <X_k, y> for k < m (deterministic two-stage reduction), X_k FP64, y FP64/FP32.
+template <class TY>
+void multi_dot(const double* const* X, int m, const TY* y, int64_t n, double* d_out, double* d_part,
+               hipStream_t st);
+// W = sum_k u[k] V_k ; R = sum_k u[k] HV_k - theta W ; d_out[0]=|W|^2, d_out[1]=|R|^2
+template <class TH>
+void ritz_combine(const double* const* V, const TH* const* HV, int m, const double* d_u,
+                  const double* d_theta, double* W, double* R, int64_t n, double* d_out,
+                  double* d_part, hipStream_t st);
+void scale(double* x, const double* d_norm2, int64_t n, hipStream_t st);  // x /= sqrt(norm2)
+// t = (R/|R|) / (theta - d), with the 1e-12 regularisation of davidson.h:855-862
+void precondition(const double* R, const double* d_rn2, const double* hii, const double* d_theta,
+                  double* t, int64_t n, hipStream_t st);
+// t -= sum_k s[k] V_k and returns |t|^2 in d_out[0]
+void gs_update(const double* const* V, int m, const double* d_s, double* t, int64_t n,
+               double* d_out, double* d_part, hipStream_t st);
+// Y_j = sum_k U[k + ld*j] X_k, j < mk (thick restart rotation); Y may not alias X.
+template <class T>
+void rotate(const T* const* X, int m, const double* d_U, int ld, int mk, T* const* Y, int64_t n,
+            hipStream_t st);
+// symmetric eigensolve of the m x m column-major d_H (ld) on the device: ascending
+// eigenvalues in d_E, eigenvectors in the columns of d_U (ld).
+void syev_device(const double* d_H, int m, int ld, double* d_E, double* d_U, hipStream_t st);
+void density(const double* W, const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, int L,
+             double* d_dens, double* d_part, hipStream_t st);
+
+// RDMs (rdm.hip)
+struct RdmPlan {
+  int64_t na = 0, nb = 0;
+  int L = 0;
+  const uint64_t* a = nullptr;
+  const uint64_t* b = nullptr;
+  const Table* sa = nullptr;
+  const Table* da = nullptr;
+  const Table* sb = nullptr;
+  const Table* db = nullptr;
+  const AbTable* ab = nullptr;
+};
+// one[2][L^2], two[4][L^4] in the reference layout (correlation.h:194-444) from the
+// separated-convention vector W.
+void rdm_compute(const RdmPlan& P, const double* W, double* d_one, double* d_two, hipStream_t st);
+// Hellmann-Feynman gradient: dh[p*L+q] = g*(one_a+one_b)[p+L*q];
+// deri[((p*L+q)*L+r)*L+s] = g/2 * sum_blk two[blk][p + L*r + L^2*q + L^3*s]
+void rdm_backward(const double* d_one, const double* d_two, int L, double g, double* d_dh,
+                  double* d_deri, hipStream_t st);
+
+}  // namespace sbdg

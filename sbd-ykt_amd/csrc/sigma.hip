@@ -1,0 +1,331 @@
+// sigma = H c on the alpha x beta product basis (replaces mult, tpb/mult.h:376-632).
+//
+// In the separated phase order H splits into
+//   sigma[ia,ib] = hii*c                                                   (diagonal)
+//                + sum_{ja in S(ia)+D(ia)} va(ia,ja) c[ja,ib]             (alpha product)
+//                + sum_{jb in S(ib)+D(ib)} vb(ib,jb) c[ia,jb]             (beta product)
+//                + sum_{ja in S(ia)} sa [ sum_{k in occ(ib)} (ai|kk) c[ja,ib]
+//                                        + sum_{jb in S(ib)} sb (ai|bj) c[ja,jb] ]
+//                + sum_{jb in S(ib)} sb [ sum_{k in occ(ia)} (kk|bj) ] c[ia,jb]   (alpha-beta)
+// where va, vb are the string-pair values of the excitation tables.  The alpha product
+// streams whole rows c[ja,:] (coalesced, one workgroup per row tile, no atomics); the
+// beta product is the same kernel on the transposed vector; the alpha-beta kernel owns
+// one bra row per workgroup, stages the ket row c[ja,:] in LDS chunk by chunk together
+// with the (ai|..) slice, and gathers from LDS.  Every output element has one owner.
+#include <algorithm>
+
+#include "sbdg_internal.h"
+
+namespace sbdg {
+
+namespace {
+
+template <class T>
+struct V16;
+template <>
+struct V16<double> {
+  using t = double2;
+  static constexpr int n = 2;
+};
+template <>
+struct V16<float> {
+  using t = float4;
+  static constexpr int n = 4;
+};
+
+constexpr int TS = 256;  // threads of the row-streaming product
+
+// out[r, tile] = (DIAG ? hii*X : 0)[r, tile] + sum_q val[q] X[col[q], tile]
+template <class T, bool DIAG, bool VECTOR>
+__global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, const int64_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ col, const T* __restrict__ val,
+                                             const T* __restrict__ X, const T* __restrict__ hii,
+                                             T* __restrict__ out) {
+  constexpr int VN = V16<T>::n;
+  using VT = typename V16<T>::t;
+  const int64_t bid = blockIdx.x;
+  const int64_t tile = bid / nrows;
+  const int64_t r = bid - tile * nrows;
+  const int64_t c0 = tile * (int64_t)(TS * VN) + (int64_t)threadIdx.x * VN;
+  if (c0 >= ncols) return;
+  T acc[VN];
+  const bool full = VECTOR && (c0 + VN <= ncols);
+  if (DIAG) {
+    const T* h = hii + r * ncols + c0;
+    const T* x = X + r * ncols + c0;
+    if (full) {
+      T hv[VN], xv[VN];
+      *reinterpret_cast<VT*>(hv) = *reinterpret_cast<const VT*>(h);
+      *reinterpret_cast<VT*>(xv) = *reinterpret_cast<const VT*>(x);
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc[k] = hv[k] * xv[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc[k] = (c0 + k < ncols) ? h[k] * x[k] : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < VN; ++k) acc[k] = T(0);
+  }
+  int64_t q = rowptr[r];
+  const int64_t q1 = rowptr[r + 1];
+  if (full) {
+    for (; q + 4 <= q1; q += 4) {
+      const int32_t j0 = col[q], j1 = col[q + 1], j2 = col[q + 2], j3 = col[q + 3];
+      const T v0 = val[q], v1 = val[q + 1], v2 = val[q + 2], v3 = val[q + 3];
+      T x0[VN], x1[VN], x2[VN], x3[VN];
+      *reinterpret_cast<VT*>(x0) = *reinterpret_cast<const VT*>(X + (int64_t)j0 * ncols + c0);
+      *reinterpret_cast<VT*>(x1) = *reinterpret_cast<const VT*>(X + (int64_t)j1 * ncols + c0);
+      *reinterpret_cast<VT*>(x2) = *reinterpret_cast<const VT*>(X + (int64_t)j2 * ncols + c0);
+      *reinterpret_cast<VT*>(x3) = *reinterpret_cast<const VT*>(X + (int64_t)j3 * ncols + c0);
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        acc[k] += v0 * x0[k];
+        acc[k] += v1 * x1[k];
+        acc[k] += v2 * x2[k];
+        acc[k] += v3 * x3[k];
+      }
+    }
+    for (; q < q1; ++q) {
+      const T v = val[q];
+      T x[VN];
+      *reinterpret_cast<VT*>(x) = *reinterpret_cast<const VT*>(X + (int64_t)col[q] * ncols + c0);
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc[k] += v * x[k];
+    }
+    *reinterpret_cast<VT*>(out + r * ncols + c0) = *reinterpret_cast<VT*>(acc);
+  } else {
+    for (; q < q1; ++q) {
+      const T v = val[q];
+      const T* x = X + (int64_t)col[q] * ncols + c0;
+#pragma unroll
+      for (int k = 0; k < VN; ++k)
+        if (c0 + k < ncols) acc[k] += v * x[k];
+    }
+#pragma unroll
+    for (int k = 0; k < VN; ++k)
+      if (c0 + k < ncols) out[r * ncols + c0 + k] = acc[k];
+  }
+}
+
+constexpr int TT = 64;  // transpose tile
+
+// out[c][r] = in[r][c]   (ADD: out[c][r] += in[r][c])
+template <class T, bool ADD>
+__global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T* __restrict__ in,
+                                                   T* __restrict__ out) {
+  __shared__ T tile[TT][TT + 1];
+  const int64_t r0 = (int64_t)blockIdx.y * TT, c0 = (int64_t)blockIdx.x * TT;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int k = ty; k < TT; k += 4) {
+    const int64_t r = r0 + k, c = c0 + tx;
+    if (r < R && c < C) tile[k][tx] = in[r * C + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < TT; k += 4) {
+    const int64_t c = c0 + k, r = r0 + tx;
+    if (r < R && c < C) {
+      if (ADD) out[c * R + r] += tile[tx][k];
+      else out[c * R + r] = tile[tx][k];
+    }
+  }
+}
+
+// alpha-beta and cross-spin single terms; one workgroup per bra row ia, thread t owns
+// columns ib = t + m*blockDim.x, m < M.  LDS: [chunk of c[ja,:]] [L*L slice].
+template <class T, int M>
+__global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, const uint64_t* __restrict__ astr,
+                                             const uint64_t* __restrict__ bstr,
+                                             const int64_t* __restrict__ rpSa,
+                                             const int32_t* __restrict__ colSa,
+                                             const uint32_t* __restrict__ excSa,
+                                             const uint32_t* __restrict__ pack,
+                                             const int32_t* __restrict__ cptr, int nch, int64_t chunk,
+                                             const T* __restrict__ eri, const T* __restrict__ c,
+                                             T* __restrict__ sigma) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* cb = reinterpret_cast<T*>(smem);
+  T* Vs = cb + chunk;
+  const int NT = blockDim.x;
+  const int64_t ia = blockIdx.x;
+  const int64_t LL = (int64_t)L * L;
+  T acc[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) acc[m] = T(0);
+  const int64_t q0 = rpSa[ia], q1 = rpSa[ia + 1];
+  for (int64_t t = q0 - 1; t < q1; ++t) {
+    const bool self = (t < q0);
+    int64_t ja = ia;
+    T coef = T(1);
+    int ai = 0;
+    if (!self) {
+      ja = colSa[t];
+      const uint32_t e = excSa[t];
+      ai = (int)((e >> 6) & 63u) * L + (int)(e & 63u);  // a*L + i
+      coef = ((e >> 24) & 1u) ? T(-1) : T(1);
+    }
+    __syncthreads();
+    if (self) {
+      const uint64_t A = astr[ia];
+      for (int x = threadIdx.x; x < LL; x += NT) {
+        T v = T(0);
+        uint64_t y = A;
+        while (y) {
+          const int k = __builtin_ctzll(y);
+          y &= y - 1;
+          v += eri[((int64_t)k * L + k) * LL + x];
+        }
+        Vs[x] = v;
+      }
+    } else {
+      const T* src = eri + (int64_t)ai * LL;
+      for (int x = threadIdx.x; x < LL; x += NT) Vs[x] = src[x];
+    }
+    __syncthreads();
+    if (!self) {
+      const T* crow = c + ja * nb;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int64_t ib = threadIdx.x + (int64_t)m * NT;
+        if (ib < nb) {
+          uint64_t y = bstr[ib];
+          T d = T(0);
+          while (y) {
+            const int k = __builtin_ctzll(y);
+            y &= y - 1;
+            d += Vs[k * L + k];
+          }
+          acc[m] += coef * d * crow[ib];
+        }
+      }
+    }
+    for (int k = 0; k < nch; ++k) {
+      const int64_t j0 = (int64_t)k * chunk;
+      const int64_t j1 = min(nb, j0 + chunk);
+      __syncthreads();
+      for (int64_t x = threadIdx.x; x < j1 - j0; x += NT) cb[x] = c[ja * nb + j0 + x];
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int64_t ib = threadIdx.x + (int64_t)m * NT;
+        if (ib < nb) {
+          const int32_t* cp = cptr + ib * (nch + 1) + k;
+          const int e0 = cp[0], e1 = cp[1];
+          T s = T(0);
+          for (int e = e0; e < e1; ++e) {
+            const uint32_t p = pack[e];
+            const int jb = (int)(p & 0x3FFFFu) - (int)j0;
+            const T v = Vs[(p >> 18) & 0xFFFu];
+            s += (((p >> 30) & 1u) ? -v : v) * cb[jb];
+          }
+          acc[m] += coef * s;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int64_t ib = threadIdx.x + (int64_t)m * NT;
+    if (ib < nb) sigma[ia * nb + ib] += acc[m];
+  }
+}
+
+template <class T>
+void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, const T* X, const T* hii,
+                 T* out, hipStream_t st) {
+  if (nrows == 0 || ncols == 0) return;
+  constexpr int VN = V16<T>::n;
+  const int64_t ntiles = (ncols + TS * VN - 1) / (TS * VN);
+  const int64_t nblk = ntiles * nrows;
+  if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
+  const T* val = nullptr;
+  if constexpr (sizeof(T) == 8) val = reinterpret_cast<const T*>(tb.val.p);
+  else val = reinterpret_cast<const T*>(tb.val32.p);
+  const bool vec = (ncols % VN) == 0;
+  dim3 g((unsigned)nblk), b(TS);
+  if (diag) {
+    if (vec) hipLaunchKernelGGL((k_spmm<T, true, true>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
+    else hipLaunchKernelGGL((k_spmm<T, true, false>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
+  } else {
+    if (vec) hipLaunchKernelGGL((k_spmm<T, false, true>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
+    else hipLaunchKernelGGL((k_spmm<T, false, false>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
+  }
+}
+
+template <class T, bool ADD>
+void launch_transpose(int64_t R, int64_t C, const T* in, T* out, hipStream_t st) {
+  if (R == 0 || C == 0) return;
+  dim3 g((unsigned)((C + TT - 1) / TT), (unsigned)((R + TT - 1) / TT));
+  hipLaunchKernelGGL((k_transpose<T, ADD>), g, dim3(256), 0, st, R, C, in, out);
+}
+
+template <class T, int M>
+void launch_ab_m(const SigmaPlan& P, int nt, size_t lds, const T* eri, const T* c, T* sigma, hipStream_t st) {
+  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)P.na), dim3(nt), lds, st, P.nb, P.L, P.a, P.b,
+                     P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, P.ab->pack.p, P.ab->cptr.p, P.ab->nch,
+                     P.ab->chunk, eri, c, sigma);
+}
+
+template <class T>
+void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream_t st) {
+  if (P.na == 0 || P.nb == 0) return;
+  int nt = (int)std::min<int64_t>(1024, ((P.nb + 63) / 64) * 64);
+  const int64_t need = (P.nb + nt - 1) / nt;
+  const size_t lds = ab_lds_bytes(P.L, P.ab->chunk, (int)sizeof(T));
+  if (need <= 1) launch_ab_m<T, 1>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 2) launch_ab_m<T, 2>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 4) launch_ab_m<T, 4>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 8) launch_ab_m<T, 8>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 12) launch_ab_m<T, 12>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 16) launch_ab_m<T, 16>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 24) launch_ab_m<T, 24>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 32) launch_ab_m<T, 32>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 40) launch_ab_m<T, 40>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 48) launch_ab_m<T, 48>(P, nt, lds, eri, c, sigma, st);
+  else if (need <= 64) launch_ab_m<T, 64>(P, nt, lds, eri, c, sigma, st);
+  else throw Error(SBDG_ERR_LIMIT, "beta string set beyond 65536 strings");
+}
+
+}  // namespace
+
+void transpose_d(int64_t R, int64_t C, const double* in, double* out, hipStream_t st) {
+  launch_transpose<double, false>(R, C, in, out, st);
+  SBDG_CHECK(hipGetLastError());
+}
+
+size_t ab_lds_bytes(int L, int64_t chunk, int elem) {
+  return (size_t)elem * ((size_t)chunk + (size_t)L * L);
+}
+
+template <class T>
+void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream_t st, hipEvent_t* ev) {
+  const int64_t N = P.na * P.nb;
+  T* cT = scratch;
+  T* sT = scratch + N;
+  const T* hii;
+  const T* eri;
+  if constexpr (sizeof(T) == 8) {
+    hii = reinterpret_cast<const T*>(P.hii);
+    eri = reinterpret_cast<const T*>(P.eri);
+  } else {
+    hii = reinterpret_cast<const T*>(P.hii32);
+    eri = reinterpret_cast<const T*>(P.eri32);
+  }
+  if (ev) SBDG_CHECK(hipEventRecord(ev[0], st));
+  launch_transpose<T, false>(P.na, P.nb, c, cT, st);
+  if (ev) SBDG_CHECK(hipEventRecord(ev[1], st));
+  launch_spmm<T>(false, P.nb, P.na, *P.mb, cT, nullptr, sT, st);
+  if (ev) SBDG_CHECK(hipEventRecord(ev[2], st));
+  launch_spmm<T>(true, P.na, P.nb, *P.ma, c, hii, sigma, st);
+  if (ev) SBDG_CHECK(hipEventRecord(ev[3], st));
+  launch_transpose<T, true>(P.nb, P.na, sT, sigma, st);
+  if (ev) SBDG_CHECK(hipEventRecord(ev[4], st));
+  launch_ab<T>(P, eri, c, sigma, st);
+  if (ev) SBDG_CHECK(hipEventRecord(ev[5], st));
+  SBDG_CHECK(hipGetLastError());
+}
+
+template void sigma_apply<double>(const SigmaPlan&, const double*, double*, double*, hipStream_t, hipEvent_t*);
+template void sigma_apply<float>(const SigmaPlan&, const float*, float*, float*, hipStream_t, hipEvent_t*);
+
+}  // namespace sbdg

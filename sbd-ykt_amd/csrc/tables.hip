@@ -1,0 +1,387 @@
+// Excitation tables and the diagonal, built on the device.
+//
+// Tables (replacing GenerateExcitation + MakeSmartHelper, tpb/helper.h:132-197,367-437):
+// for every bra string the kets at XOR-popcount 2 (singles) and 4 (doubles), ascending.
+// One workgroup per bra row scans the string set twice (count, then fill with a
+// ballot/prefix so the ascending order survives), then a per-entry pass derives the
+// orbitals with ctz and the separated-order sign with a masked popcount, and evaluates
+// the spin-string-only part of the Slater-Condon element (determinants.h:423-465).
+#include <algorithm>
+
+#include "sbdg_internal.h"
+
+namespace sbdg {
+
+namespace {
+
+constexpr int TB = 256;  // threads per row workgroup
+
+__device__ inline uint64_t between_mask(int x, int y) {
+  // bits strictly between orbitals x and y (x != y)
+  int lo = x < y ? x : y, hi = x < y ? y : x;
+  uint64_t below_hi = (hi >= 64) ? ~0ull : ((1ull << hi) - 1ull);
+  uint64_t upto_lo = (lo + 1 >= 64) ? ~0ull : ((1ull << (lo + 1)) - 1ull);
+  return below_hi & ~upto_lo;
+}
+
+__device__ inline int block_sum(int v, int* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  int t = 0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+    sh[0] = t;
+  }
+  __syncthreads();
+  t = sh[0];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(TB) k_count(const uint64_t* __restrict__ s, int64_t n,
+                                              int64_t* __restrict__ cs, int64_t* __restrict__ cd) {
+  __shared__ int sh[TB / 64];
+  const int64_t i = blockIdx.x;
+  const uint64_t si = s[i];
+  int ns = 0, nd = 0;
+  for (int64_t j = threadIdx.x; j < n; j += TB) {
+    int d = __popcll(si ^ s[j]);
+    ns += (d == 2);
+    nd += (d == 4);
+  }
+  ns = block_sum(ns, sh);
+  nd = block_sum(nd, sh);
+  if (threadIdx.x == 0) {
+    cs[i] = ns;
+    cd[i] = nd;
+  }
+}
+
+// exclusive scan of cnt[0..n) into rp[0..n] (one workgroup, fixed order)
+__global__ void __launch_bounds__(1024) k_scan(const int64_t* __restrict__ cnt, int64_t n,
+                                               int64_t* __restrict__ rp) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t b = t * per, e = min(n, b + per);
+  int64_t s = 0;
+  for (int64_t k = b; k < e; ++k) s += cnt[k];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t acc = 0;
+    for (int k = 0; k < 1024; ++k) {
+      int64_t v = part[k];
+      part[k] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int64_t acc = part[t];
+  for (int64_t k = b; k < e; ++k) {
+    rp[k] = acc;
+    acc += cnt[k];
+  }
+  if (e == n && b < e) rp[n] = acc;
+  if (n == 0 && t == 0) rp[0] = 0;
+}
+
+__global__ void __launch_bounds__(TB) k_fill(const uint64_t* __restrict__ s, int64_t n,
+                                             const int64_t* __restrict__ rps,
+                                             const int64_t* __restrict__ rpd,
+                                             int32_t* __restrict__ cols, int32_t* __restrict__ cold) {
+  __shared__ int wcs[TB / 64], wcd[TB / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t i = blockIdx.x;
+  const uint64_t si = s[i];
+  int64_t ps = rps[i], pd = rpd[i];
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int64_t base = 0; base < n; base += TB) {
+    const int64_t j = base + threadIdx.x;
+    int d = (j < n) ? __popcll(si ^ s[j]) : 0;
+    uint64_t ms = __ballot(d == 2), md = __ballot(d == 4);
+    if (lane == 0) {
+      wcs[wid] = __popcll(ms);
+      wcd[wid] = __popcll(md);
+    }
+    __syncthreads();
+    int offs = 0, offd = 0, tots = 0, totd = 0;
+    for (int w = 0; w < TB / 64; ++w) {
+      if (w < wid) {
+        offs += wcs[w];
+        offd += wcd[w];
+      }
+      tots += wcs[w];
+      totd += wcd[w];
+    }
+    if (d == 2) cols[ps + offs + __popcll(ms & lt)] = (int32_t)j;
+    if (d == 4) cold[pd + offd + __popcll(md & lt)] = (int32_t)j;
+    ps += tots;
+    pd += totd;
+    __syncthreads();
+  }
+}
+
+__device__ inline double eri4(const double* __restrict__ eri, int L, int p, int q, int r, int s) {
+  return eri[(((int64_t)p * L + q) * L + r) * L + s];
+}
+
+// singles: bra-only a, ket-only i; sign from the ket string between them;
+// value sign*(h_ai + sum over occ(ket)\{i} of [(ai|kk) - (ak|ki)])
+__global__ void __launch_bounds__(TB) k_single(const uint64_t* __restrict__ s, int L,
+                                               const double* __restrict__ h1,
+                                               const double* __restrict__ eri,
+                                               const int64_t* __restrict__ rp,
+                                               const int32_t* __restrict__ col,
+                                               uint32_t* __restrict__ exc, double* __restrict__ val,
+                                               float* __restrict__ val32) {
+  const int64_t r = blockIdx.x;
+  const uint64_t bra = s[r];
+  for (int64_t q = rp[r] + threadIdx.x; q < rp[r + 1]; q += TB) {
+    const uint64_t ket = s[col[q]];
+    const uint64_t bo = bra & ~ket, ko = ket & ~bra;
+    const int a = __builtin_ctzll(bo), i = __builtin_ctzll(ko);
+    const int neg = __popcll(ket & between_mask(i, a)) & 1;
+    double v = h1[a * L + i];
+    uint64_t occ = ket & ~(1ull << i);
+    while (occ) {
+      int k = __builtin_ctzll(occ);
+      occ &= occ - 1;
+      v += eri4(eri, L, a, i, k, k) - eri4(eri, L, a, k, k, i);
+    }
+    v = neg ? -v : v;
+    exc[q] = (uint32_t)i | ((uint32_t)a << 6) | ((uint32_t)neg << 24);
+    val[q] = v;
+    val32[q] = (float)v;
+  }
+}
+
+// doubles: ket-only I<J, bra-only A<B; sign by the two-pair rule of TwoExcite
+// (determinants.h:447-465) applied to the ket string; value sign*((AI|BJ)-(AJ|BI))
+__global__ void __launch_bounds__(TB) k_double(const uint64_t* __restrict__ s, int L,
+                                               const double* __restrict__ eri,
+                                               const int64_t* __restrict__ rp,
+                                               const int32_t* __restrict__ col,
+                                               uint32_t* __restrict__ exc, double* __restrict__ val,
+                                               float* __restrict__ val32) {
+  const int64_t r = blockIdx.x;
+  const uint64_t bra = s[r];
+  for (int64_t q = rp[r] + threadIdx.x; q < rp[r + 1]; q += TB) {
+    const uint64_t ket = s[col[q]];
+    uint64_t bo = bra & ~ket, ko = ket & ~bra;
+    const int A = __builtin_ctzll(bo);
+    bo &= bo - 1;
+    const int B = __builtin_ctzll(bo);
+    const int I = __builtin_ctzll(ko);
+    ko &= ko - 1;
+    const int J = __builtin_ctzll(ko);
+    int neg = (__popcll(ket & between_mask(I, A)) + __popcll(ket & between_mask(J, B))) & 1;
+    if (A > J || B < I) neg ^= 1;
+    double v = eri4(eri, L, A, I, B, J) - eri4(eri, L, A, J, B, I);
+    v = neg ? -v : v;
+    exc[q] = (uint32_t)I | ((uint32_t)J << 6) | ((uint32_t)A << 12) | ((uint32_t)B << 18) |
+             ((uint32_t)neg << 24);
+    val[q] = v;
+    val32[q] = (float)v;
+  }
+}
+
+__global__ void k_merge(int64_t n, const int64_t* __restrict__ rps, const int32_t* __restrict__ cs,
+                        const double* __restrict__ vs, const float* __restrict__ vs32,
+                        const int64_t* __restrict__ rpd, const int32_t* __restrict__ cd,
+                        const double* __restrict__ vd, const float* __restrict__ vd32,
+                        int64_t* __restrict__ rpm, int32_t* __restrict__ cm,
+                        double* __restrict__ vm, float* __restrict__ vm32) {
+  const int64_t r = blockIdx.x;
+  const int64_t b = rps[r] + rpd[r];
+  const int64_t ns = rps[r + 1] - rps[r], nd = rpd[r + 1] - rpd[r];
+  if (threadIdx.x == 0) {
+    rpm[r] = b;
+    if (r == n - 1) rpm[n] = rps[n] + rpd[n];
+  }
+  for (int64_t k = threadIdx.x; k < ns + nd; k += blockDim.x) {
+    if (k < ns) {
+      cm[b + k] = cs[rps[r] + k];
+      vm[b + k] = vs[rps[r] + k];
+      vm32[b + k] = vs32[rps[r] + k];
+    } else {
+      cm[b + k] = cd[rpd[r] + k - ns];
+      vm[b + k] = vd[rpd[r] + k - ns];
+      vm32[b + k] = vd32[rpd[r] + k - ns];
+    }
+  }
+}
+
+__global__ void k_abpack(int64_t nnz, int L, const int32_t* __restrict__ col,
+                         const uint32_t* __restrict__ exc, uint32_t* __restrict__ pack) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  const uint32_t e = exc[q];
+  const uint32_t j = e & 63u, b = (e >> 6) & 63u, neg = (e >> 24) & 1u;
+  pack[q] = (uint32_t)col[q] | ((b * (uint32_t)L + j) << 18) | (neg << 30);
+}
+
+// cptr[r*(nch+1)+k] = first entry of row r with ket >= k*chunk (k = 0..nch)
+__global__ void k_abchunk(int64_t n, int nch, int64_t chunk, const int64_t* __restrict__ rp,
+                          const int32_t* __restrict__ col, int32_t* __restrict__ cptr) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * (nch + 1)) return;
+  const int64_t r = t / (nch + 1);
+  const int k = (int)(t % (nch + 1));
+  const int64_t lim = (int64_t)k * chunk;
+  int64_t lo = rp[r], hi = rp[r + 1];
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)col[mid] < lim) lo = mid + 1;
+    else hi = mid;
+  }
+  cptr[t] = (int32_t)lo;
+}
+
+// per-string diagonal parts: E[s] = sum_{i in s} h_ii + sum_{i<j in s} [(ii|jj)-(ij|ji)]
+__global__ void k_strdiag(const uint64_t* __restrict__ s, int64_t n, int L,
+                          const double* __restrict__ h1, const double* __restrict__ eri,
+                          double* __restrict__ E) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint64_t x = s[t];
+  double e = 0.0;
+  while (x) {
+    int i = __builtin_ctzll(x);
+    x &= x - 1;
+    e += h1[i * L + i];
+    uint64_t y = x;
+    while (y) {
+      int j = __builtin_ctzll(y);
+      y &= y - 1;
+      e += eri4(eri, L, i, i, j, j) - eri4(eri, L, i, j, j, i);
+    }
+  }
+  E[t] = e;
+}
+
+// hii[ia,ib] = core + Ea[ia] + Eb[ib] + sum_{i in a, j in b} (ii|jj)
+__global__ void __launch_bounds__(TB) k_hii(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
+                                            int64_t nb, int L, double core,
+                                            const double* __restrict__ Ea,
+                                            const double* __restrict__ Eb,
+                                            const double* __restrict__ eri, double* __restrict__ hii) {
+  __shared__ double J[64];
+  const int64_t ia = blockIdx.x;
+  const uint64_t A = a[ia];
+  if (threadIdx.x < L) {
+    double v = 0.0;
+    uint64_t x = A;
+    while (x) {
+      int i = __builtin_ctzll(x);
+      x &= x - 1;
+      v += eri4(eri, L, i, i, threadIdx.x, threadIdx.x);
+    }
+    J[threadIdx.x] = v;
+  }
+  __syncthreads();
+  const double base = core + Ea[ia];
+  for (int64_t ib = threadIdx.x; ib < nb; ib += TB) {
+    uint64_t x = b[ib];
+    double v = base + Eb[ib];
+    while (x) {
+      int j = __builtin_ctzll(x);
+      x &= x - 1;
+      v += J[j];
+    }
+    hii[ia * nb + ib] = v;
+  }
+}
+
+inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, const double* d_eri,
+                 Table& ts, Table& td, hipStream_t st) {
+  ts.n = td.n = n;
+  DBuf<int64_t> cs, cd;
+  cs.alloc(n);
+  cd.alloc(n);
+  ts.rowptr.alloc(n + 1);
+  td.rowptr.alloc(n + 1);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_count, dim3((unsigned)n), dim3(TB), 0, st, d_str, n, cs.p, cd.p);
+    SBDG_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cs.p, n, ts.rowptr.p);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cd.p, n, td.rowptr.p);
+  SBDG_CHECK(hipGetLastError());
+  SBDG_CHECK(hipMemcpyAsync(&ts.nnz, ts.rowptr.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SBDG_CHECK(hipMemcpyAsync(&td.nnz, td.rowptr.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SBDG_CHECK(hipStreamSynchronize(st));
+  if (ts.nnz >= (int64_t)INT32_MAX || td.nnz >= (int64_t)INT32_MAX)
+    throw Error(SBDG_ERR_LIMIT, "excitation table beyond 2^31 entries");
+  for (Table* t : {&ts, &td}) {
+    t->col.alloc(std::max<int64_t>(t->nnz, 1));
+    t->exc.alloc(std::max<int64_t>(t->nnz, 1));
+    t->val.alloc(std::max<int64_t>(t->nnz, 1));
+    t->val32.alloc(std::max<int64_t>(t->nnz, 1));
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)n), dim3(TB), 0, st, d_str, n, ts.rowptr.p, td.rowptr.p,
+                       ts.col.p, td.col.p);
+    hipLaunchKernelGGL(k_single, dim3((unsigned)n), dim3(TB), 0, st, d_str, L, d_h1, d_eri, ts.rowptr.p,
+                       ts.col.p, ts.exc.p, ts.val.p, ts.val32.p);
+    hipLaunchKernelGGL(k_double, dim3((unsigned)n), dim3(TB), 0, st, d_str, L, d_eri, td.rowptr.p, td.col.p,
+                       td.exc.p, td.val.p, td.val32.p);
+    SBDG_CHECK(hipGetLastError());
+  }
+  SBDG_CHECK(hipStreamSynchronize(st));
+}
+
+void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st) {
+  m.n = s.n;
+  m.nnz = s.nnz + d.nnz;
+  m.rowptr.alloc(m.n + 1);
+  m.col.alloc(std::max<int64_t>(m.nnz, 1));
+  m.val.alloc(std::max<int64_t>(m.nnz, 1));
+  m.val32.alloc(std::max<int64_t>(m.nnz, 1));
+  if (m.n > 0) {
+    hipLaunchKernelGGL(k_merge, dim3((unsigned)m.n), dim3(TB), 0, st, m.n, s.rowptr.p, s.col.p, s.val.p,
+                       s.val32.p, d.rowptr.p, d.col.p, d.val.p, d.val32.p, m.rowptr.p, m.col.p, m.val.p,
+                       m.val32.p);
+    SBDG_CHECK(hipGetLastError());
+  } else {
+    SBDG_CHECK(hipMemsetAsync(m.rowptr.p, 0, sizeof(int64_t), st));
+  }
+}
+
+void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st) {
+  if (sb.n > (1 << 18)) throw Error(SBDG_ERR_LIMIT, "beta string set beyond 2^18 strings");
+  ab.chunk = chunk;
+  ab.nch = (int)((sb.n + chunk - 1) / chunk);
+  if (ab.nch < 1) ab.nch = 1;
+  ab.pack.alloc(std::max<int64_t>(sb.nnz, 1));
+  ab.cptr.alloc((size_t)sb.n * (ab.nch + 1) + 1);
+  if (sb.nnz > 0)
+    hipLaunchKernelGGL(k_abpack, dim3(blocks_for(sb.nnz, 256)), dim3(256), 0, st, sb.nnz, L, sb.col.p,
+                       sb.exc.p, ab.pack.p);
+  const int64_t nt = sb.n * (ab.nch + 1);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_abchunk, dim3(blocks_for(nt, 256)), dim3(256), 0, st, sb.n, ab.nch, chunk,
+                       sb.rowptr.p, sb.col.p, ab.cptr.p);
+  SBDG_CHECK(hipGetLastError());
+}
+
+void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
+                 const double* d_h1, const double* d_eri, double* d_hii, hipStream_t st) {
+  DBuf<double> Ea, Eb;
+  Ea.alloc(std::max<int64_t>(na, 1));
+  Eb.alloc(std::max<int64_t>(nb, 1));
+  if (na > 0) hipLaunchKernelGGL(k_strdiag, dim3(blocks_for(na, 256)), dim3(256), 0, st, d_a, na, L, d_h1, d_eri, Ea.p);
+  if (nb > 0) hipLaunchKernelGGL(k_strdiag, dim3(blocks_for(nb, 256)), dim3(256), 0, st, d_b, nb, L, d_h1, d_eri, Eb.p);
+  if (na > 0 && nb > 0)
+    hipLaunchKernelGGL(k_hii, dim3((unsigned)na), dim3(TB), 0, st, d_a, d_b, nb, L, core, Ea.p, Eb.p, d_eri, d_hii);
+  SBDG_CHECK(hipGetLastError());
+  SBDG_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace sbdg

@@ -1,0 +1,77 @@
+"""Shared test helpers: fixtures -> problems, RDM comparisons, energy identity."""
+import gzip
+import os
+
+import numpy as np
+
+from conftest import GOLDEN, load_golden
+from sbdg import fcidump as fio
+from sbdg import synthetic as syn
+
+SMALL_CASES = ("asym_L8", "asym_L14", "sym_L10")
+
+
+def small_case(name):
+    g = load_golden(f"ref_{name}.npz")
+    return g, dict(norb=int(g["norb"]), core=float(g["core"]), h1=g["h1"], eri=g["eri"],
+                   astr=g["astr"], bstr=g["bstr"])
+
+
+def tables_from_ref(flat, na, nb):
+    """The reference's helper dump: per table the row lengths, then the entries."""
+    out, pos = [], 0
+    for n in (na, na, nb, nb):
+        lens = flat[pos:pos + n]
+        pos += n
+        rp = np.zeros(n + 1, dtype=np.int64)
+        rp[1:] = np.cumsum(lens)
+        cols = flat[pos:pos + rp[-1]]
+        pos += rp[-1]
+        out.append((rp, cols))
+    assert pos == len(flat)
+    return out
+
+
+def dataset(tag):
+    """Reference dataset (data/n2 or data/h2o) with the threshold-1e-3 strings."""
+    path = os.path.join(GOLDEN, f"{tag}_fcidump.txt.gz")
+    tmp = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"sbdg_{tag}_{os.getpid()}.txt")
+    with gzip.open(path, "rb") as f, open(tmp, "wb") as g:
+        g.write(f.read())
+    try:
+        header, vals, idx = fio.load_fcidump(tmp)
+    finally:
+        os.remove(tmp)
+    strings = fio.load_alpha_strings(os.path.join(GOLDEN, f"{tag}_1em3-alpha.txt"))
+    return header, vals, idx, strings
+
+
+def energy_from_rdm(core, h1, eri, one, two):
+    """E = I0 + sum h_ij gamma[i+Lj] + 1/2 sum (ik|jl) Gamma[i+Lj+L^2k+L^3l]
+    (samples/selected_basis_diagonalization/main.cc:166-216)."""
+    L = h1.shape[0]
+    g = (one[0] + one[1]).reshape(L, L, order="F")      # g[i,j] = gamma[i + L j]
+    G = two.sum(0).reshape(L, L, L, L, order="F")       # G[i,j,k,l]
+    e1 = np.einsum("ij,ij->", h1, g)
+    e2 = 0.5 * np.einsum("ikjl,ijkl->", eri, G)
+    return core + e1 + e2
+
+
+def rdm_close(a1, a2, b1, b2, tol, mixed_sum=False):
+    """1-RDM, same-spin blocks, and either every block or blocks 1+2 summed."""
+    assert np.abs(a1 - b1).max() < tol, np.abs(a1 - b1).max()
+    for k in (0, 3):
+        assert np.abs(a2[k] - b2[k]).max() < tol, (k, np.abs(a2[k] - b2[k]).max())
+    if mixed_sum:
+        assert np.abs((a2[1] + a2[2]) - (b2[1] + b2[2])).max() < tol
+    else:
+        for k in (1, 2):
+            assert np.abs(a2[k] - b2[k]).max() < tol, (k, np.abs(a2[k] - b2[k]).max())
+
+
+def workload_small(L, na_e, nb_e, na, nb, seed):
+    rng = np.random.default_rng(seed)
+    h1, eri = syn.synthetic_integrals(L, rng)
+    a = syn.sample_strings(L, na_e, na, rng) if na else syn.full_space(L, na_e)
+    b = syn.sample_strings(L, nb_e, nb, rng) if nb else syn.full_space(L, nb_e)
+    return dict(norb=L, core=0.25, h1=h1, eri=eri, astr=a, bstr=b)

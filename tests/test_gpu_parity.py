@@ -1,0 +1,348 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle and the reference goldens.
+
+Bars: integer work (tables, orbitals, signs) bit-exact; FP64 sigma within 1e-12 of the
+oracle relative to its largest element; FP64 energy within 1e-8 Ha and RDMs within
+1e-7 of the reference; FP32 sigma within 2e-5 relative; mixed-precision energy within
+1e-6 Ha of FP64.
+"""
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import oracle
+import sbdg
+from conftest import load_golden
+from helpers import SMALL_CASES, dataset, energy_from_rdm, rdm_close, small_case, tables_from_ref, workload_small
+from sbdg import fcidump as fio
+from sbdg import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def solver(case, **opt):
+    return sbdg.Solver(case["norb"], case["core"], case["h1"], case["eri"], case["astr"], case["bstr"], **opt)
+
+
+def oproblem(case):
+    return oracle.Problem(case["norb"], case["core"], case["h1"], case["astr"], case["bstr"], eri=case["eri"])
+
+
+def sep_sign(a, b):
+    """s(A,B) = (-1)^(sum over p in A of #{q in B: q < p}): interleaved vs separated order."""
+    t = 0
+    A, B = int(a), int(b)
+    p = 0
+    while A >> p:
+        if (A >> p) & 1:
+            t += bin(B & ((1 << p) - 1)).count("1")
+        p += 1
+    return -1.0 if t & 1 else 1.0
+
+
+def medium_case():
+    return workload_small(26, 5, 5, 400, 300, 31)
+
+
+# ---------------------------------------------------------------- integer work
+
+@pytest.mark.parametrize("name", SMALL_CASES)
+def test_tables_bit_exact_vs_reference(name):
+    g, case = small_case(name)
+    s = solver(case)
+    ref = tables_from_ref(g["tables"], s.na, s.nb)
+    for which in range(4):
+        rp, col, exc, val = s.table(which)
+        assert np.array_equal(rp, ref[which][0]), which
+        assert np.array_equal(col.astype(np.int64), ref[which][1]), which
+
+
+def test_tables_bit_exact_vs_oracle_medium():
+    case = medium_case()
+    s, p = solver(case), oproblem(case)
+    for which in range(4):
+        rp, col, _, _ = s.table(which)
+        rpo, colo = p.table(which)
+        assert np.array_equal(rp, rpo) and np.array_equal(col.astype(np.int64), colo)
+
+
+def test_pair_orbitals_and_signs():
+    """Orbitals in every table entry, and the sign of every reconstructed matrix element
+    (separated-order value times s(bra) s(ket)) equal the reference convention's."""
+    case = workload_small(12, 4, 3, 200, 150, 41)
+    s, p = solver(case), oproblem(case)
+    rng = np.random.default_rng(0)
+    A, B, L = p.a, p.b, case["norb"]
+    for which in range(4):
+        rp, col, exc, val = s.table(which)
+        strs = A if which < 2 else B
+        rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        for q in rng.choice(len(col), size=min(400, len(col)), replace=False):
+            bra, ket = int(strs[rows[q]]), int(strs[col[q]])
+            e = int(exc[q])
+            if which % 2 == 0:
+                i, a = e & 63, (e >> 6) & 63
+                assert (ket & ~bra) == 1 << i and (bra & ~ket) == 1 << a
+            else:
+                I, J, Aa, Bb = e & 63, (e >> 6) & 63, (e >> 12) & 63, (e >> 18) & 63
+                assert (ket & ~bra) == (1 << I) | (1 << J) and I < J
+                assert (bra & ~ket) == (1 << Aa) | (1 << Bb) and Aa < Bb
+            # the element in the reference convention for a sample of spectator strings
+            for other in rng.choice(len(B if which < 2 else A), size=3, replace=False):
+                if which < 2:
+                    ab, bb, ak, bk = bra, int(B[other]), ket, int(B[other])
+                else:
+                    ab, bb, ak, bk = int(A[other]), bra, int(A[other]), ket
+                href = p.hij(np.array([ab], np.uint64), np.array([bb], np.uint64),
+                             np.array([ak], np.uint64), np.array([bk], np.uint64))[0]
+                hsep = val[q]
+                if which % 2 == 0:  # singles: add the other-spin occupation term
+                    i, a = e & 63, (e >> 6) & 63
+                    occ = bb if which == 0 else ab
+                    neg = (e >> 24) & 1
+                    extra = sum(case["eri"][a, i, k, k] for k in range(L) if (occ >> k) & 1)
+                    hsep = hsep + (-extra if neg else extra)
+                hgpu = sep_sign(ab, bb) * sep_sign(ak, bk) * hsep
+                assert abs(hgpu - href) <= 1e-12 * max(1.0, abs(href))
+                if abs(href) > 1e-10:
+                    assert np.sign(hgpu) == np.sign(href)
+
+
+# ---------------------------------------------------------------- sigma
+
+@pytest.mark.parametrize("name", SMALL_CASES)
+def test_hii_and_sigma_vs_reference(name):
+    g, case = small_case(name)
+    s = solver(case)
+    assert np.abs(s.hii() - g["hii"]).max() < 1e-13
+    vec = np.random.default_rng(int(g["vec_seed"])).normal(size=s.ndet)
+    sig = s.sigma(vec)
+    assert np.abs(sig - g["sigma"]).max() <= 1e-12 * np.abs(g["sigma"]).max()
+    sig32 = s.sigma(vec, precision=1)
+    assert np.abs(sig32 - g["sigma"]).max() <= 2e-5 * np.abs(g["sigma"]).max()
+
+
+def test_sigma_columns_are_matrix_columns():
+    """sigma(e_J) is column J of H: every element (sign included) of those columns."""
+    case = workload_small(8, 3, 2, 30, 20, 51)
+    s = solver(case)
+    H = bf.hamiltonian(case["norb"], case["core"], case["h1"], case["eri"], case["astr"], case["bstr"])
+    for J in (0, 1, 37, 199, s.ndet - 1):
+        e = np.zeros(s.ndet)
+        e[J] = 1.0
+        col = s.sigma(e)
+        assert np.abs(col - H[:, J]).max() < 1e-13
+        assert np.array_equal(np.sign(np.round(col, 12)), np.sign(np.round(H[:, J], 12)))
+
+
+def test_sigma_medium_vs_oracle_and_symmetry():
+    case = medium_case()
+    s, p = solver(case), oproblem(case)
+    rng = np.random.default_rng(2)
+    x, y = rng.normal(size=s.ndet), rng.normal(size=s.ndet)
+    sx, ox = s.sigma(x), p.sigma(x)
+    assert np.abs(sx - ox).max() <= 1e-12 * np.abs(ox).max()
+    sy = s.sigma(y)
+    assert abs(x @ sy - y @ sx) < 1e-10 * abs(x @ sy)            # H symmetric
+    sxy = s.sigma(2.0 * x - 3.0 * y)
+    assert np.abs(sxy - (2.0 * sx - 3.0 * sy)).max() < 1e-11 * np.abs(sxy).max()  # linear
+
+
+def test_sigma_c1_shape_row_sample():
+    """Full c1 shape (norb=26, 5+5, 4000 x 4000 strings): GPU tables equal the oracle's,
+    and sigma on a sample of rows equals the oracle restricted to those rows."""
+    wl = syn.config_workload("c1_L26_5e5e_4000")
+    case = dict(norb=wl.norb, core=wl.core, h1=wl.h1, eri=wl.eri, astr=wl.astr, bstr=wl.bstr)
+    s, p = solver(case), oproblem(case)
+    for which in range(4):
+        rp, col, _, _ = s.table(which)
+        rpo, colo = p.table(which)
+        assert np.array_equal(rp, rpo) and np.array_equal(col.astype(np.int64), colo)
+    x = np.random.default_rng(3).normal(size=s.ndet)
+    sx = s.sigma(x)
+    hii = p.hii()
+    for r0 in (0, 1999):
+        o = p.sigma_rows(x, hii, r0, r0 + 2)
+        blk = slice(r0 * s.nb, (r0 + 2) * s.nb)
+        assert np.abs(sx[blk] - o[blk]).max() <= 1e-12 * np.abs(o[blk]).max()
+
+
+# ---------------------------------------------------------------- Davidson, RDMs
+
+@pytest.mark.parametrize("name", SMALL_CASES)
+def test_energy_rdm_density_vs_reference(name):
+    g, case = small_case(name)
+    s = solver(case, max_it=200, max_nb=10, eps=float(g["tol"]), do_rdm=1)
+    e = s.run()
+    assert abs(e - float(g["energy"])) < 1e-8
+    one, two = s.rdm()
+    rdm_close(one, two, g["rdm1"], g["rdm2"], 1e-7, mixed_sum=True)
+    L = case["norb"]
+    dens = np.empty(2 * L)
+    dens[0::2] = one[0].reshape(L, L).diagonal()
+    dens[1::2] = one[1].reshape(L, L).diagonal()
+    assert np.abs(dens - g["density"]).max() < 1e-7
+    assert np.abs(s.density() - g["density"]).max() < 1e-7
+    assert abs(energy_from_rdm(case["core"], case["h1"], case["eri"], one, two) - e) < 1e-9
+    w = s.vector()
+    assert abs(np.linalg.norm(w) - 1.0) < 1e-12
+    assert abs(abs(w @ g["wf"]) - 1.0) < 1e-7
+
+
+def test_rdm_all_blocks_vs_bruteforce():
+    """Every block, the mixed-spin ones included, against second quantisation."""
+    for case in (workload_small(6, 3, 2, 12, 9, 21), workload_small(6, 2, 2, None, None, 22)):
+        s = solver(case, max_it=50, max_nb=12, eps=1e-11, do_rdm=1)
+        s.run()
+        one, two = s.rdm()
+        one_b, two_b = bf.rdms(case["norb"], s.astr, s.bstr, s.vector())
+        rdm_close(one, two, one_b, two_b, 1e-9, mixed_sum=False)
+
+
+def test_dataset_n2_energy():
+    header, vals, idx, strings = dataset("n2")
+    g = load_golden("ref_n2_1em3.npz")
+    L, core, h1, eri = fio.integrals_from_fcidump(header, vals, idx)
+    s = sbdg.Solver(L, core, h1, eri, strings, strings, max_it=100, max_nb=10, eps=1e-8, do_rdm=1)
+    e = s.run()
+    assert abs(e - float(g["energy"])) < 1e-8
+    one, two = s.rdm()
+    assert np.abs(one - g["rdm1"]).max() < 1e-7
+    assert abs(np.einsum("bijij->", two.reshape(4, L, L, L, L)) - float(g["rdm2_sum_trace"])) < 1e-7
+    # the published table value through the six-digit path (data/n2/README.md:8)
+    Lr, core_r, h1r, erir = fio.integrals_from_fcidump(header, fio.round_like_broadcast(vals), idx)
+    sr = sbdg.Solver(Lr, core_r, h1r, erir, strings, strings, max_it=100, max_nb=10, eps=1e-8)
+    er = sr.run()
+    assert abs(er - float(g["energy_rounded"])) < 1e-8 and round(er, 6) == -109.041511
+
+
+def test_dataset_h2o_energy():
+    header, vals, idx, strings = dataset("h2o")
+    g = load_golden("ref_h2o_1em3.npz")
+    L, core, h1, eri = fio.integrals_from_fcidump(header, vals, idx)
+    s = sbdg.Solver(L, core, h1, eri, strings, strings, max_it=100, max_nb=10, eps=1e-8)
+    e = s.run()
+    assert abs(e - float(g["energy"])) < 1e-8
+    Lr, core_r, h1r, erir = fio.integrals_from_fcidump(header, fio.round_like_broadcast(vals), idx)
+    er = sbdg.Solver(Lr, core_r, h1r, erir, strings, strings, max_it=100, max_nb=10, eps=1e-8).run()
+    assert abs(er - float(g["energy_rounded"])) < 1e-8 and round(er, 7) == -76.2359376
+
+
+def test_c0_energy_and_rdm_vs_reference():
+    g = load_golden("ref_c0.npz")
+    wl = syn.config_workload("c0_L12_4e4e_full")
+    s = sbdg.Solver(wl.norb, wl.core, wl.h1, wl.eri, wl.astr, wl.bstr, max_it=200, max_nb=10, eps=1e-8, do_rdm=1)
+    e = s.run()
+    assert abs(e - float(g["energy"])) < 1e-8
+    one, two = s.rdm()
+    rdm_close(one, two, g["rdm1"], g["rdm2"], 1e-7, mixed_sum=True)
+    vec = np.random.default_rng(0).normal(size=s.ndet)
+    sig = s.sigma(vec)
+    assert np.abs(sig[:4096] - g["sigma_head"]).max() < 1e-11
+    assert abs(float(sig @ vec) - float(g["sigma_dot_vec"])) < 1e-9 * abs(float(g["sigma_dot_vec"]))
+
+
+def test_thick_restart_and_warm_start():
+    case = medium_case()
+    e1 = solver(case, max_it=100, max_nb=8, eps=1e-9).run()
+    s2 = solver(case, max_it=100, max_nb=12, restart_keep=4, eps=1e-9)
+    e2 = s2.run()
+    assert abs(e1 - e2) < 1e-9
+    w = s2.vector()
+    s3 = solver(case, max_it=10, max_nb=8, eps=1e-8, init=2)
+    s3.set_vector(w)
+    e3 = s3.run()
+    assert abs(e3 - e2) < 1e-9 and s3.stats()["iterations"] <= 2
+
+
+def test_mixed_precision_energy():
+    case = medium_case()
+    e64 = solver(case, max_it=100, max_nb=12, restart_keep=4, eps=1e-8).run()
+    s = solver(case, max_it=100, max_nb=12, restart_keep=4, eps=1e-6, precision=1, refresh_k=4)
+    e = s.run()
+    st = s.stats()
+    assert st["sigma_fp32"] > 0
+    assert abs(e - e64) < 1e-6
+
+
+def test_hellmann_feynman_backward_vs_finite_differences():
+    case = workload_small(8, 3, 3, 40, 40, 61)
+    L = case["norb"]
+    s = solver(case, max_it=100, max_nb=12, eps=1e-11, do_rdm=1)
+    s.run()
+    dh, de = s.rdm_backward(1.0)
+    rng = np.random.default_rng(5)
+    step = 1e-4
+
+    def energy(h1, eri):
+        return solver(dict(case, h1=h1, eri=eri), max_it=100, max_nb=12, eps=1e-11).run()
+
+    for _ in range(4):
+        p, q = rng.integers(0, L, size=2)
+        dH = np.zeros((L, L))
+        dH[p, q] += 1.0
+        dH[q, p] = 1.0 if p != q else dH[p, q]
+        fd = (energy(case["h1"] + step * dH, case["eri"]) - energy(case["h1"] - step * dH, case["eri"])) / (2 * step)
+        an = np.sum(dh * dH)
+        assert abs(fd - an) < 1e-6, (p, q, fd, an)
+    for _ in range(4):
+        p, q, r, t = rng.integers(0, L, size=4)
+        dE = np.zeros((L, L, L, L))
+        for a, b, c, d in ((p, q, r, t), (q, p, r, t), (p, q, t, r), (q, p, t, r),
+                           (r, t, p, q), (t, r, p, q), (r, t, q, p), (t, r, q, p)):
+            dE[a, b, c, d] = 1.0
+        fd = (energy(case["h1"], case["eri"] + step * dE) - energy(case["h1"], case["eri"] - step * dE)) / (2 * step)
+        an = np.sum(de * dE)
+        assert abs(fd - an) < 1e-6, (p, q, r, t, fd, an)
+
+
+def test_autograd_function():
+    import torch
+    case = workload_small(8, 3, 3, 40, 40, 61)
+    s = solver(case, max_it=100, max_nb=12, eps=1e-11)
+    h1 = torch.tensor(case["h1"], requires_grad=True)
+    eri = torch.tensor(case["eri"], requires_grad=True)
+    e = sbdg.ground_state_energy(h1, eri, s, case["core"])
+    e.backward()
+    one, two = s.rdm()
+    L = case["norb"]
+    assert torch.allclose(h1.grad, torch.tensor((one[0] + one[1]).reshape(L, L, order="F")))
+    # energy identity through the gradients: E = core + <h, dE/dh> + <eri, dE/deri>
+    ident = case["core"] + float((h1.grad * h1).sum()) + float((eri.grad * eri).sum())
+    assert abs(ident - float(e)) < 1e-9
+
+
+def test_carryover_matches_row_weights():
+    case = medium_case()
+    s = solver(case, max_it=100, max_nb=10, eps=1e-8, threshold=0.01)
+    s.run()
+    w = s.vector().reshape(s.na, s.nb)
+    D = (w * w).sum(1)
+    order = np.argsort(-D, kind="stable")
+    S = np.cumsum(D[order])
+    kept = int(np.searchsorted(S, 1 - 0.01, side="right")) + 1
+    co, _ = s.carryover()
+    assert len(co) == kept and np.array_equal(co, s.astr[order[:kept]])
+    co2, tw = s.carryover(ratio=0.1)
+    assert len(co2) == int(0.1 * s.na) and abs(tw - (1 - D[order[:len(co2)]].sum())) < 1e-12
+
+
+def test_errors_from_the_host():
+    case = workload_small(6, 3, 2, 12, 9, 21)
+    with pytest.raises(sbdg.SbdgError) as ex:
+        solver(dict(case, astr=np.concatenate([case["astr"], case["astr"][:1]])))
+    assert ex.value.code == sbdg.ERR_ARG
+    with pytest.raises(sbdg.SbdgError):
+        solver(dict(case, bstr=np.array([0b111, 0b11], dtype=np.uint64)))
+    with pytest.raises(sbdg.SbdgError):
+        solver(case, max_nb=0)
+    s = solver(case)
+    with pytest.raises(sbdg.SbdgError) as ex:
+        s.vector()
+    assert ex.value.code == sbdg.ERR_STATE
+    with pytest.raises(sbdg.SbdgError):
+        s.set_options(init=2)
+        s.run()
+    # max_time ~ 0 stops after the first step with an unconverged vector
+    s = solver(case, max_it=50, max_nb=8, eps=1e-12, max_time=1e-9)
+    s.run()
+    assert s.stats()["iterations"] == 1 and s.stats()["converged"] == 0

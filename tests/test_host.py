@@ -1,0 +1,118 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every symbol its
+header declares, the FCIDUMP / string loaders keep the reference's semantics, and the
+synthetic workload generator is deterministic and follows its specification."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import sbdg
+from conftest import ROOT
+from helpers import dataset
+from sbdg import fcidump as fio
+from sbdg import synthetic as syn
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "sbdg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sbdg_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    l = C.CDLL(sbdg.LIBPATH)
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(l, s), s
+    # the Python binding declares a signature for every one of them, and nothing else
+    assert sorted(sbdg.SIGNATURES) == syms
+
+
+def test_binding_loads_and_rejects_missing_device():
+    l = sbdg.lib()
+    o = sbdg.default_options()
+    assert (o.max_it, o.max_nb, o.eps, o.init, o.do_rdm, o.threshold) == (1, 10, 1e-4, 0, 0, 0.01)
+    assert o.max_time == 86400.0 and o.restart_keep == 1 and o.precision == 0
+    # no GPU in a CPU-only run: creating a context on a missing device yields NULL
+    import torch
+    if not torch.cuda.is_available():
+        assert not l.sbdg_create(0)
+    assert not l.sbdg_create(1 << 20)
+
+
+def test_fcidump_loader_matches_reference_setup():
+    header, vals, idx, strings = dataset("n2")
+    assert header["NORB"] == "18" and header["NELEC"] == "14"
+    L, core, h1, eri = fio.integrals_from_fcidump(header, vals, idx)
+    I0, h1o, packed = oracle.setup_from_fcidump(L, vals, idx)
+    assert core == I0 == 25.92968333508
+    assert np.array_equal(h1, h1o)
+    assert np.array_equal(oracle.pack_eri(eri), packed)
+    # 8-fold symmetry of the expansion
+    assert np.array_equal(eri, eri.transpose(1, 0, 2, 3))
+    assert np.array_equal(eri, eri.transpose(2, 3, 0, 1))
+    assert len(strings) == 239 and strings[0] == (1 << 7) - 1
+    assert np.all(np.diff(strings.astype(np.int64)) > 0)
+
+
+def test_fcidump_roundtrip_and_rounding(tmp_path):
+    rng = np.random.default_rng(4)
+    h1, eri = syn.synthetic_integrals(5, rng)
+    f = tmp_path / "fcidump.txt"
+    fio.write_fcidump(str(f), 5, 4, h1, eri, core=1.25)
+    header, vals, idx = fio.load_fcidump(str(f))
+    L, core, h1b, erib = fio.integrals_from_fcidump(header, vals, idx)
+    assert L == 5 and core == 1.25
+    assert np.array_equal(h1, h1b) and np.array_equal(eri, erib)
+    r = fio.round_like_broadcast(np.array([2.323747945954686, -7.283619520573128e-05, 25.92968333508]))
+    assert list(r) == [2.32375, -7.28362e-05, 25.9297]
+
+
+def test_string_file_io(tmp_path):
+    s = np.array([0b0111, 0b1011, 0b0111, 0b1101], dtype=np.uint64)
+    f = tmp_path / "a.txt"
+    fio.write_strings(str(f), s, 6)
+    assert open(f).read().split() == ["000111", "001011", "000111", "001101"]
+    got = fio.load_alpha_strings(str(f))
+    assert list(got) == [0b0111, 0b1011, 0b1101]
+
+
+@pytest.mark.parametrize("name", list(syn.CONFIGS))
+def test_config_shapes(name):
+    cfg = syn.CONFIGS[name]
+    if cfg.get("na", 0) and cfg["na"] > 5000:
+        pytest.skip("large sampled set: covered by the generator property test")
+    wl = syn.config_workload(name)
+    L, na_e = cfg["norb"], cfg["nalpha"]
+    if "na" in cfg:
+        assert len(wl.astr) == cfg["na"] and len(wl.bstr) == cfg["nb"]
+    else:
+        from math import comb
+        assert len(wl.astr) == comb(L, na_e)
+    for s in (wl.astr, wl.bstr):
+        assert s[0] == (1 << na_e) - 1
+        assert np.all(np.diff(s.astype(np.float64)) > 0)
+        assert all(bin(int(x)).count("1") == na_e for x in s)
+    assert np.allclose(np.diag(wl.h1), np.linspace(-2.0, 1.0, L))
+    assert np.array_equal(wl.h1, wl.h1.T)
+    e = wl.eri
+    for perm in ((1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1)):
+        assert np.array_equal(e, e.transpose(perm))
+    M = e.reshape(L * L, L * L)
+    assert np.linalg.eigvalsh(M).min() > -1e-10  # positive semidefinite pair matrix
+
+
+def test_generator_deterministic_and_levels():
+    a1 = syn.sample_strings(26, 5, 600, np.random.default_rng(9))
+    a2 = syn.sample_strings(26, 5, 600, np.random.default_rng(9))
+    assert np.array_equal(a1, a2)
+    hf = (1 << 5) - 1
+    lv = np.array([bin(int(x) & ~hf).count("1") for x in a1])
+    # level weights 0.5^k: level 1 is exhausted (5*21 = 105 strings) and level 2 dominates
+    assert (lv == 0).sum() == 1 and (lv == 1).sum() <= 105 and (lv == 2).sum() > (lv == 4).sum()
+    with pytest.raises(ValueError):
+        syn.sample_strings(6, 3, 21, np.random.default_rng(0))
