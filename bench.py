@@ -1,0 +1,181 @@
+"""Benchmark of the selected-basis Davidson hot path on one MI355X.
+
+Workload: norb=36, 9 alpha + 9 beta electrons, 20,000 x 20,000 sampled strings
+(400M determinants), seed 2 -- the configuration the mixed-vs-FP64 comparison is quoted
+on.  Synthetic integrals and strings (no datasets available offline).
+
+A step is one sigma build (H c over all determinants, FP64) on vectors already resident
+in HBM.  `value` is determinants x steps / seconds (det*H*v/s) over the K timed steps.
+Also reported:
+  * FP32 sigma throughput of the mixed path;
+  * time-to-|dE| < 1e-6 Ha for the FP64 Davidson and the mixed Davidson, against the
+    converged FP64 energy, and the mixed energy error;
+  * roofline of the dominant sigma kernel: its algorithmic HBM bytes per launch (model
+    below) over its average duration from HIP events on the library stream;
+  * cpu_baseline: the CPU restatement (oracle) timed on this host's cores on a bounded
+    sample of rows of the same workload.
+
+Run: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--no-tts] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sbd-ykt_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(na, nb, nnz, s):
+    """Algorithmic HBM bytes per launch of each sigma kernel (s = bytes per element).
+
+    transpose:    read c, write c^T                         2 N s
+    beta product: stream c^T rows: nnz_b rows of na, write  s na (nnzSb+nnzDb) + N s + 12 nnz_b
+    alpha product: hii, c, nnz_a rows of nb, write          s nb (nnzSa+nnzDa) + 3 N s + 12 nnz_a
+    transpose-add: read sigma^T and sigma, write sigma      3 N s
+    alpha-beta:   one ket row per (ia, ja) pair incl. ia    s nb (na + nnzSa) + 2 N s + 4 nnzSb
+    """
+    N = na * nb
+    sa, da, sb, db = nnz
+    return {
+        "transpose": 2 * N * s,
+        "beta": s * na * (sb + db) + N * s + 12 * (sb + db),
+        "alpha": s * nb * (sa + da) + 3 * N * s + 12 * (sa + da),
+        "transpose_add": 3 * N * s,
+        "alpha_beta": s * nb * (na + sa) + 2 * N * s + 4 * sb,
+    }
+
+
+def survey_b_alg(na, nb, nnz, s):
+    """Per-sigma algorithmic bytes of the row-streamed model:
+    N 3s + s Nb (nnzSa + nnzDa) + s Nb nnzSa + 4 nnz(tables)."""
+    sa, da, sb, db = nnz
+    return na * nb * 3 * s + s * nb * (sa + da) + s * nb * sa + 4 * (sa + da + sb + db)
+
+
+def time_to_de(traj_t, traj_e, e_ref, tol=1e-6):
+    for t, e in zip(traj_t, traj_e):
+        if abs(e - e_ref) < tol:
+            return float(t)
+    return None
+
+
+def cpu_baseline(wl, rows, budget_s=20.0):
+    """Oracle sigma on a row sample (all columns of each sampled row) on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle
+
+    p = oracle.Problem(wl.norb, wl.core, wl.h1, wl.astr, wl.bstr, eri=wl.eri)
+    N = p.ndet
+    x = np.random.default_rng(0).random(N, dtype=np.float32).astype(np.float64)
+    done, t_tot, r0 = 0, 0.0, 0
+    while done < rows and t_tot < budget_s:
+        k = min(rows - done, max(1, rows // 8))
+        t0 = time.perf_counter()
+        p.sigma_rows(x, None, r0, r0 + k)
+        t_tot += time.perf_counter() - t0
+        done += k
+        r0 += k
+        print(f"[bench] cpu baseline: {done} rows in {t_tot:.1f} s", flush=True)
+    return dict(value=done * p.nb / t_tot, unit="det\u00b7H\u00b7v/s", cores=oracle.lib().orc_num_threads(),
+                kind="port", sample=f"{done} alpha rows x {p.nb} beta strings of the same workload, "
+                                    f"one FP64 sigma row block, {t_tot:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2_L36_9e9e_20000")
+    ap.add_argument("--no-tts", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import sbdg
+    from sbdg import synthetic as syn
+
+    t0 = time.perf_counter()
+    wl = syn.config_workload(args.config)
+    print(f"[bench] workload {args.config}: {len(wl.astr)} x {len(wl.bstr)} = {wl.ndet} determinants "
+          f"(generated in {time.perf_counter() - t0:.1f} s)", flush=True)
+    s = sbdg.Solver(wl.norb, wl.core, wl.h1, wl.eri, wl.astr, wl.bstr)
+    s.build()
+    st = s.stats()
+    na, nb = len(wl.astr), len(wl.bstr)
+    nnz = st["nnz"]
+    print(f"[bench] tables+diagonal {st['t_tables']:.2f} s, nnz {nnz}, terms/sigma {st['terms']:.3e}", flush=True)
+
+    # warmup, then K timed FP64 sigma builds (library events per kernel; wall clock overall)
+    s.bench_sigma(0, max(1, args.warmup))
+    t0 = time.perf_counter()
+    ms = s.bench_sigma(0, args.steps)
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    value = wl.ndet * args.steps / wall
+    print(f"[bench] fp64 sigma {ms_step:.2f} ms/step (events {ms[0]:.2f} ms)", flush=True)
+    s.bench_sigma(1, max(1, args.warmup))
+    ms32 = s.bench_sigma(1, args.steps)
+    print(f"[bench] fp32 sigma {ms32[0]:.2f} ms/step", flush=True)
+
+    names = ["transpose", "beta", "alpha", "transpose_add", "alpha_beta"]
+    kb = kernel_bytes(na, nb, nnz, 8)
+    per = {n: float(ms[i + 1]) for i, n in enumerate(names)}
+    dom = max(per, key=per.get)
+    achieved = kb[dom] / (per[dom] * 1e-3) / 1e9
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    launch_ms=round(per[dom], 3), alg_bytes=int(kb[dom]))
+    b_alg = survey_b_alg(na, nb, nnz, 8)
+    sigma_model = dict(alg_bytes=int(b_alg), achieved_GBs=round(b_alg / (ms[0] * 1e-3) / 1e9, 1),
+                       frac=round(b_alg / (ms[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       kernel_ms={k: round(v, 3) for k, v in per.items()})
+
+    tts = None
+    if not args.no_tts:
+        s.set_options(max_it=40, max_nb=12, restart_keep=4, eps=1e-5, precision=0, max_time=600.0)
+        e64 = s.run()
+        t64, tr64 = s.trajectory()
+        st64 = s.stats()
+        s.set_options(precision=1, refresh_k=4, eps=1e-5)
+        emx = s.run()
+        tmx, trmx = s.trajectory()
+        stmx = s.stats()
+        a = time_to_de(t64, tr64, e64)
+        b = time_to_de(tmx, trmx, e64)
+        tts = dict(e_fp64=e64, e_mixed=emx, mixed_error=abs(emx - e64), fp64_s=a, mixed_s=b,
+                   speedup=(a / b) if (a and b) else None, fp64_sigma=st64["sigma_fp64"],
+                   mixed_sigma_fp32=stmx["sigma_fp32"], mixed_sigma_fp64=stmx["sigma_fp64"],
+                   fp64_davidson_s=st64["t_davidson"], mixed_davidson_s=stmx["t_davidson"])
+        print(f"[bench] time-to-|dE|<1e-6: fp64 {a} s, mixed {b} s; mixed error {abs(emx - e64):.2e}", flush=True)
+    s.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(wl, args.cpu_rows)
+
+    line = {
+        "metric": "Davidson time-to-|dE|<1e-6 Ha, mixed vs FP64 on 1 GPU; det\u00b7H\u00b7v/s",
+        "value": value, "unit": "det\u00b7H\u00b7v/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "norb=36, 9+9 electrons, 20000 x 20000 sampled strings, seed 2",
+                   "determinants": wl.ndet, "terms_per_sigma": st["terms"], "name": args.config},
+        "fp32_sigma": {"value": wl.ndet / (ms32[0] * 1e-3), "ms_per_step": float(ms32[0]), "dtype": "f32"},
+        "time_to_solution": tts,
+        "roofline": roofline,
+        "sigma_model": sigma_model,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

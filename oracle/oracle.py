@@ -144,7 +144,7 @@ class Problem:
     def sigma_rows(self, c, hii, ia0, ia1):
         c = np.ascontiguousarray(c, dtype=np.float64)
         out = np.zeros(self.ndet)
-        lib().orc_sigma_rows(self._p, _d(hii), _d(c), _d(out), ia0, ia1)
+        lib().orc_sigma_rows(self._p, None if hii is None else _d(hii), _d(c), _d(out), ia0, ia1)
         return out
 
     def term_count(self):

@@ -379,7 +379,8 @@ void orc_sigma(const orc_problem *p, const double *hii, const double *T, double 
 }
 
 /* sigma restricted to bra rows [ia0, ia1): the bounded CPU sample used as a timing
- * baseline.  Writes Wb rows ia0..ia1-1 (Wb indexed like the full vector). */
+ * baseline.  Writes Wb rows ia0..ia1-1 (Wb indexed like the full vector); hii may be
+ * NULL, then the diagonal is evaluated in place. */
 void orc_sigma_rows(const orc_problem *p, const double *hii, const double *T, double *Wb,
                     int64_t ia0, int64_t ia1) {
   const int64_t nb = p->nb;
@@ -388,7 +389,7 @@ void orc_sigma_rows(const orc_problem *p, const double *hii, const double *T, do
   for (int64_t ia = ia0; ia < ia1; ++ia) {
     for (int64_t ib = 0; ib < nb; ++ib) {
       u128 I = interleave(p->a[ia], p->b[ib], L);
-      double acc = hii[ia * nb + ib] * T[ia * nb + ib];
+      double acc = (hii ? hii[ia * nb + ib] : zero_excite(p, I)) * T[ia * nb + ib];
       for (int64_t q = p->rpSa[ia]; q < p->rpSa[ia + 1]; ++q) {
         int64_t ja = p->Sa[q];
         acc += hij(p, I, interleave(p->a[ja], p->b[ib], L)) * T[ja * nb + ib];

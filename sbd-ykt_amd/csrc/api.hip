@@ -26,23 +26,6 @@
 
 using namespace sbdg;
 
-namespace sbdg {
-void multi_dot_impl(const double* X, int64_t ldx, int m, const double* y, int64_t n, double* out, double* part,
-                    hipStream_t st);
-void multi_dot_impl(const double* X, int64_t ldx, int m, const float* y, int64_t n, double* out, double* part,
-                    hipStream_t st);
-template <class TH>
-void ritz_combine_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, int m, const double* u,
-                       const double* theta, double* W, double* R, int64_t n, double* out, double* part,
-                       hipStream_t st);
-void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st);
-void precond_impl(const double* R, const double* rn2, const double* hii, const double* theta, double* t, int64_t n,
-                  hipStream_t st);
-void gs_impl(const double* V, int64_t ldv, int m, const double* s, double* t, int64_t n, double* out, double* part,
-             hipStream_t st);
-template <class T>
-void rotate_impl(T* X, int64_t ldx, int m, const double* U, int ld, int mk, int64_t n, hipStream_t st);
-}  // namespace sbdg
 
 namespace {
 constexpr int MAXSUB = 48;
@@ -71,6 +54,8 @@ struct sbdg_ctx {
   DBuf<double> V, HV, W, R, T, HX, scr64, small;
   DBuf<float> HV32, c32, s32, scr32;
   DBuf<double> d_one, d_two, d_dens;
+  DBuf<double> bx64, by64;  // benchmark vectors (filled once on the device)
+  DBuf<float> bx32, by32;
   sbdg_stats stats{};
   std::vector<double> traj_t, traj_e;
   double energy = 0.0;
@@ -847,29 +832,26 @@ int sbdg_sigma(sbdg_ctx* c, const double* x, double* y, int precision) {
 
 int sbdg_bench_sigma(sbdg_ctx* c, int precision, int reps, double* ms) {
   return guard(c, [&]() {
-    if (!ms || reps < 1) return fail(c, SBDG_ERR_ARG, "bad arguments");
+    if (!ms || reps < 1 || (precision != 0 && precision != 1)) return fail(c, SBDG_ERR_ARG, "bad arguments");
     ensure_built(c);
     const int64_t N = (int64_t)c->a.size() * c->b.size();
+    if (c->bx64.n != (size_t)N) {
+      c->bx64.alloc(N);
+      fill_test_vector(c->bx64.p, N, 7, c->st);
+    }
+    if (precision == 1 && c->bx32.n != (size_t)N) {
+      c->bx32.alloc(N);
+      cast_d2f(c->bx64.p, c->bx32.p, N, c->st);
+    }
+    if (precision == 0) c->by64.alloc(N);
+    else c->by32.alloc(N);
+    SBDG_CHECK(hipStreamSynchronize(c->st));
     hipEvent_t ev[6];
     for (auto& e : ev) SBDG_CHECK(hipEventCreate(&e));
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    std::mt19937_64 g(7);
-    std::normal_distribution<double> nd;
-    std::vector<double> h(N);
-    for (auto& x : h) x = nd(g);
-    DBuf<double> x64, y64;
-    DBuf<float> x32, y32;
-    x64.alloc(N);
-    y64.alloc(N);
-    SBDG_CHECK(hipMemcpy(x64.p, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-    if (precision == 1) {
-      x32.alloc(N);
-      y32.alloc(N);
-      cast_d2f(x64.p, x32.p, N, c->st);
-    }
     for (int r = 0; r < reps; ++r) {
-      if (precision == 1) sigma32(c, x32.p, y32.p, ev);
-      else sigma64(c, x64.p, y64.p, ev);
+      if (precision == 1) sigma32(c, c->bx32.p, c->by32.p, ev);
+      else sigma64(c, c->bx64.p, c->by64.p, ev);
       SBDG_CHECK(hipEventSynchronize(ev[5]));
       float t;
       SBDG_CHECK(hipEventElapsedTime(&t, ev[0], ev[5]));

@@ -65,6 +65,17 @@ __global__ void k_sign(const uint64_t* __restrict__ a, const uint64_t* __restric
   out[t] = (par & 1) ? -in[t] : in[t];
 }
 
+// deterministic pseudo-random values in [-1, 1) (splitmix64 of the index)
+__global__ void k_fill(double* __restrict__ x, int64_t n, uint64_t seed) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = (uint64_t)t + seed * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[t] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
 __global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
     out[t] = (float)in[t];
@@ -320,6 +331,12 @@ void sign_transform(const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb
   const int64_t N = na * nb;
   if (N == 0) return;
   hipLaunchKernelGGL(k_sign, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, a, b, nb, N, in, out);
+  SBDG_CHECK(hipGetLastError());
+}
+
+void fill_test_vector(double* x, int64_t n, uint64_t seed, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n, seed);
   SBDG_CHECK(hipGetLastError());
 }
 

@@ -125,27 +125,29 @@ size_t ab_lds_bytes(int L, int64_t chunk, int elem);
 void sign_transform(const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, const double* in,
                     double* out, hipStream_t st);
 void cast_d2f(const double* in, float* out, int64_t n, hipStream_t st);
+void fill_test_vector(double* x, int64_t n, uint64_t seed, hipStream_t st);
 void cast_f2d(const float* in, double* out, int64_t n, hipStream_t st);
-// out[k] = <X_k, y> for k < m (deterministic two-stage reduction), X_k FP64, y FP64/FP32.
-template <class TY>
-void multi_dot(const double* const* X, int m, const TY* y, int64_t n, double* d_out, double* d_part,
-               hipStream_t st);
-// W = sum_k u[k] V_k ; R = sum_k u[k] HV_k - theta W ; d_out[0]=|W|^2, d_out[1]=|R|^2
+// Basis vectors are contiguous: X_k = X + k*ldx.
+// out[k] = <X_k, y> for k < m (deterministic two-stage reduction); y FP64 or FP32.
+void multi_dot_impl(const double* X, int64_t ldx, int m, const double* y, int64_t n, double* out, double* part,
+                    hipStream_t st);
+void multi_dot_impl(const double* X, int64_t ldx, int m, const float* y, int64_t n, double* out, double* part,
+                    hipStream_t st);
+// W = sum_k u[k] V_k ; R = sum_k u[k] HV_k - theta W ; out[0] = |W|^2, out[1] = |R|^2
 template <class TH>
-void ritz_combine(const double* const* V, const TH* const* HV, int m, const double* d_u,
-                  const double* d_theta, double* W, double* R, int64_t n, double* d_out,
-                  double* d_part, hipStream_t st);
-void scale(double* x, const double* d_norm2, int64_t n, hipStream_t st);  // x /= sqrt(norm2)
+void ritz_combine_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, int m, const double* u,
+                       const double* theta, double* W, double* R, int64_t n, double* out, double* part,
+                       hipStream_t st);
+void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st);  // x /= sqrt(n2[0])
 // t = (R/|R|) / (theta - d), with the 1e-12 regularisation of davidson.h:855-862
-void precondition(const double* R, const double* d_rn2, const double* hii, const double* d_theta,
-                  double* t, int64_t n, hipStream_t st);
-// t -= sum_k s[k] V_k and returns |t|^2 in d_out[0]
-void gs_update(const double* const* V, int m, const double* d_s, double* t, int64_t n,
-               double* d_out, double* d_part, hipStream_t st);
-// Y_j = sum_k U[k + ld*j] X_k, j < mk (thick restart rotation); Y may not alias X.
+void precond_impl(const double* R, const double* rn2, const double* hii, const double* theta, double* t, int64_t n,
+                  hipStream_t st);
+// t -= sum_k s[k] V_k; out[0] = |t|^2
+void gs_impl(const double* V, int64_t ldv, int m, const double* s, double* t, int64_t n, double* out, double* part,
+             hipStream_t st);
+// in place: X_j = sum_k U[k + ld*j] X_k for j < mk (thick-restart rotation)
 template <class T>
-void rotate(const T* const* X, int m, const double* d_U, int ld, int mk, T* const* Y, int64_t n,
-            hipStream_t st);
+void rotate_impl(T* X, int64_t ldx, int m, const double* U, int ld, int mk, int64_t n, hipStream_t st);
 // symmetric eigensolve of the m x m column-major d_H (ld) on the device: ascending
 // eigenvalues in d_E, eigenvectors in the columns of d_U (ld).
 void syev_device(const double* d_H, int m, int ld, double* d_E, double* d_U, hipStream_t st);
