@@ -142,7 +142,7 @@ def main():
 
     tts = None
     if not args.no_tts:
-        s.set_options(max_it=40, max_nb=12, restart_keep=4, eps=1e-5, precision=0, max_time=600.0)
+        s.set_options(max_it=40, max_nb=12, restart_keep=4, eps=1e-5, precision=0, max_time=600.0, verbose=1)
         e64 = s.run()
         t64, tr64 = s.trajectory()
         st64 = s.stats()

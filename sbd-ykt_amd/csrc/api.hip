@@ -49,7 +49,8 @@ struct sbdg_ctx {
   DBuf<uint64_t> d_a, d_b;
   Table Sa, Da, Sb, Db;
   SpmmTable Ma, Mb;
-  AbTable ab64, ab32;
+  AbTable ab64;     // beta singles by chunk (RDMs)
+  EllTable ell64, ell32;  // beta singles, ELL (alpha-beta sigma)
   // solver state
   DBuf<double> V, HV, W, R, T, HX, scr64, small;
   DBuf<float> HV32, c32, s32, scr32;
@@ -89,7 +90,7 @@ double now() {
 }
 
 int64_t choose_chunk(int64_t nb, int L, int elem) {
-  const int64_t budget = (int64_t)(LDS_BUDGET / elem) - (int64_t)L * L;
+  const int64_t budget = (int64_t)(LDS_BUDGET / elem) - (int64_t)L * L - 1;
   int64_t ch = std::max<int64_t>(256, budget);
   return std::min<int64_t>(std::max<int64_t>(nb, 1), ch);
 }
@@ -104,7 +105,7 @@ SigmaPlan plan(sbdg_ctx* c, bool fp32) {
   P.ma = &c->Ma;
   P.mb = &c->Mb;
   P.sa = &c->Sa;
-  P.ab = fp32 ? &c->ab32 : &c->ab64;
+  P.ell = fp32 ? &c->ell32 : &c->ell64;
   P.eri = c->d_eri.p;
   P.eri32 = c->d_eri32.p;
   P.hii = c->d_hii.p;
@@ -134,7 +135,8 @@ void build(sbdg_ctx* c) {
   build_spmm(c->Sa, c->Da, c->Ma, c->st);
   build_spmm(c->Sb, c->Db, c->Mb, c->st);
   build_ab(c->Sb, L, choose_chunk(nb, L, 8), c->ab64, c->st);
-  build_ab(c->Sb, L, choose_chunk(nb, L, 4), c->ab32, c->st);
+  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 8), c->ell64, c->st);
+  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 4), c->ell32, c->st);
   c->d_hii.alloc(std::max<int64_t>(N, 1));
   c->d_hii32.alloc(std::max<int64_t>(N, 1));
   compute_hii(c->d_a.p, na, c->d_b.p, nb, L, c->core, c->d_h1.p, c->d_eri.p, c->d_hii.p, c->st);
@@ -352,16 +354,10 @@ void davidson(sbdg_ctx* c) {
     ++v;
   };
 
-  if (mixed) {
-    h = 1;
-    const double ts = now();
-    refresh();
-    const double el = log_step(true);
-    if (rnorm < o.eps) converged = true;
-    else if (check_time(el, ts)) stop = true;
-    else expand_or_stop();
-  }
-  double prev_exact = theta;
+  // mixed: the first FP64 sigma comes at the first refresh; until then the projected
+  // matrix is all FP32-sigma entries
+  bool have_anchor = false;
+  double prev_exact = NAN;
   while (!converged && !stop && steps < max_steps) {
     const double ts = now();
     // sigma of the newest basis vector and column h of the projected matrix
@@ -373,8 +369,10 @@ void davidson(sbdg_ctx* c) {
       c->stats.sigma_fp32++;
       multi_dot_impl(c->V.p, N, h + 1, c->HV32.p + (int64_t)h * N, N, s.dots, s.part, c->st);
       SBDG_CHECK(hipMemcpyAsync(dots.data(), s.dots, sizeof(double) * (h + 1), hipMemcpyDeviceToHost, c->st));
-      multi_dot_impl(c->V.p + (int64_t)h * N, N, 1, c->HX.p, N, s.norms + 10, s.part, c->st);
-      SBDG_CHECK(hipMemcpyAsync(&dots[0], s.norms + 10, sizeof(double), hipMemcpyDeviceToHost, c->st));
+      if (have_anchor) {  // exact <V_h, H x> for the anchor row
+        multi_dot_impl(c->V.p + (int64_t)h * N, N, 1, c->HX.p, N, s.norms + 10, s.part, c->st);
+        SBDG_CHECK(hipMemcpyAsync(&dots[0], s.norms + 10, sizeof(double), hipMemcpyDeviceToHost, c->st));
+      }
     } else {
       timed([&] { sigma64(c, c->V.p + (int64_t)h * N, c->HV.p + (int64_t)h * N); });
       c->stats.sigma_fp64++;
@@ -421,12 +419,13 @@ void davidson(sbdg_ctx* c) {
       v = h = keep;
       ++cycle;
       refresh();
+      have_anchor = true;
       const double el = log_step(true);
       if (rnorm < o.eps) {
         converged = true;
         break;
       }
-      if (o.target_de > 0.0 && std::fabs(theta - prev_exact) < o.target_de) {
+      if (o.target_de > 0.0 && std::isfinite(prev_exact) && std::fabs(theta - prev_exact) < o.target_de) {
         converged = true;
         break;
       }

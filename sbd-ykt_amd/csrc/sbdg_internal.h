@@ -92,11 +92,31 @@ struct AbTable {
   DBuf<int32_t> cptr;
 };
 
+// Beta singles in an ELL layout for the alpha-beta sigma kernel.  For every group g of 64
+// consecutive bra columns and column chunk k there are K[g*nch+k] entries per lane, stored
+// lane-interleaved at off[g*nch+k] + e*64 + lane and padded with a zero-weight entry, so a
+// wave reads them coalesced with a wave-uniform trip count.  Entry: bits 0-17 ket column
+// offset inside the chunk (the value `chunk` addresses a zero slot: padding), bits 18-29
+// orbital pair b*L+j, bit 30 sign.  The beta-occupation part of the alpha singles rides
+// along as pseudo-entries: ket column = bra column, pair k*L+k for each occupied k.
+// Lane positions are a permutation of the columns (colperm: position -> column) sorted by
+// decreasing number of beta singles, so a 64-lane group holds columns of similar length
+// and the padding stays small when the connectivity is uneven (sampled string sets).
+struct EllTable {
+  int nch = 0;
+  int64_t chunk = 0, groups = 0;
+  DBuf<int64_t> off;  // groups*nch + 1
+  DBuf<int32_t> K;    // groups*nch
+  DBuf<uint32_t> ent;
+  DBuf<int32_t> colperm;  // groups*64 (positions beyond nb: -1)
+};
+
 // kernels (tables.hip)
 void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, const double* d_eri,
                  Table& t_single, Table& t_double, hipStream_t st);
 void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st);
 void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st);
+void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, EllTable& e, hipStream_t st);
 void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
                  const double* d_h1, const double* d_eri, double* d_hii, hipStream_t st);
 
@@ -109,7 +129,7 @@ struct SigmaPlan {
   const SpmmTable* ma = nullptr;  // alpha merged
   const SpmmTable* mb = nullptr;  // beta merged
   const Table* sa = nullptr;      // alpha singles (exc, col)
-  const AbTable* ab = nullptr;
+  const EllTable* ell = nullptr;
   const double* eri = nullptr;
   const float* eri32 = nullptr;
   const double* hii = nullptr;

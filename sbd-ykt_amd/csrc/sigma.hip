@@ -131,24 +131,31 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
   }
 }
 
-// alpha-beta and cross-spin single terms; one workgroup per bra row ia, thread t owns
-// columns ib = t + m*blockDim.x, m < M.  LDS: [chunk of c[ja,:]] [L*L slice].
+// alpha-beta and cross-spin single terms.  One workgroup per (bra row ia, tile of lane
+// positions); thread t owns positions col0 + t + m*blockDim.x, m < M (every wave owns
+// whole 64-lane groups of the ELL table), i.e. columns colperm[position].  For each ket row ja in {ia} + S(ia) the (ai|..) slice (or, for
+// ia itself, sum_{k in occ(ia)} (kk|..) with a zeroed diagonal) sits in LDS and the ket
+// row c[ja,:] streams through LDS chunk by chunk; entries are read coalesced from the ELL
+// table with wave-uniform trip counts.  LDS: [chunk + 1 (zero slot)] [L*L slice].
 template <class T, int M>
-__global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, const uint64_t* __restrict__ astr,
-                                             const uint64_t* __restrict__ bstr,
+__global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, const uint64_t* __restrict__ astr,
                                              const int64_t* __restrict__ rpSa,
                                              const int32_t* __restrict__ colSa,
                                              const uint32_t* __restrict__ excSa,
-                                             const uint32_t* __restrict__ pack,
-                                             const int32_t* __restrict__ cptr, int nch, int64_t chunk,
-                                             const T* __restrict__ eri, const T* __restrict__ c,
+                                             const int64_t* __restrict__ eoff, const int32_t* __restrict__ eK,
+                                             const uint32_t* __restrict__ ent,
+                                             const int32_t* __restrict__ colperm, int nch, int64_t chunk,
+                                             int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
                                              T* __restrict__ sigma) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* cb = reinterpret_cast<T*>(smem);
-  T* Vs = cb + chunk;
+  T* Vs = cb + chunk + 1;
   const int NT = blockDim.x;
-  const int64_t ia = blockIdx.x;
+  const int64_t ia = blockIdx.x / ntiles;
+  const int64_t col0 = (blockIdx.x - ia * ntiles) * (int64_t)NT * M;
+  const int lane = threadIdx.x & 63;
   const int64_t LL = (int64_t)L * L;
+  if (threadIdx.x == 0) cb[chunk] = T(0);
   T acc[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = T(0);
@@ -169,11 +176,13 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, const uint64_t* 
       const uint64_t A = astr[ia];
       for (int x = threadIdx.x; x < LL; x += NT) {
         T v = T(0);
-        uint64_t y = A;
-        while (y) {
-          const int k = __builtin_ctzll(y);
-          y &= y - 1;
-          v += eri[((int64_t)k * L + k) * LL + x];
+        if (x / L != x % L) {  // the diagonal (kk|jj) part is in hii already
+          uint64_t y = A;
+          while (y) {
+            const int k = __builtin_ctzll(y);
+            y &= y - 1;
+            v += eri[((int64_t)k * L + k) * LL + x];
+          }
         }
         Vs[x] = v;
       }
@@ -181,42 +190,25 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, const uint64_t* 
       const T* src = eri + (int64_t)ai * LL;
       for (int x = threadIdx.x; x < LL; x += NT) Vs[x] = src[x];
     }
-    __syncthreads();
-    if (!self) {
-      const T* crow = c + ja * nb;
-#pragma unroll
-      for (int m = 0; m < M; ++m) {
-        const int64_t ib = threadIdx.x + (int64_t)m * NT;
-        if (ib < nb) {
-          uint64_t y = bstr[ib];
-          T d = T(0);
-          while (y) {
-            const int k = __builtin_ctzll(y);
-            y &= y - 1;
-            d += Vs[k * L + k];
-          }
-          acc[m] += coef * d * crow[ib];
-        }
-      }
-    }
+    const T* crow = c + ja * nb;
     for (int k = 0; k < nch; ++k) {
       const int64_t j0 = (int64_t)k * chunk;
-      const int64_t j1 = min(nb, j0 + chunk);
+      const int64_t nj = min(nb, j0 + chunk) - j0;
       __syncthreads();
-      for (int64_t x = threadIdx.x; x < j1 - j0; x += NT) cb[x] = c[ja * nb + j0 + x];
+      for (int64_t x = threadIdx.x; x < nj; x += NT) cb[x] = crow[j0 + x];
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < M; ++m) {
-        const int64_t ib = threadIdx.x + (int64_t)m * NT;
-        if (ib < nb) {
-          const int32_t* cp = cptr + ib * (nch + 1) + k;
-          const int e0 = cp[0], e1 = cp[1];
+        const int64_t g = (col0 + (int64_t)m * NT + threadIdx.x) >> 6;  // wave-uniform
+        if (g < groups) {
+          const int64_t idx = g * nch + k;
+          const int n = eK[idx];
+          const uint32_t* ep = ent + eoff[idx] + lane;
           T s = T(0);
-          for (int e = e0; e < e1; ++e) {
-            const uint32_t p = pack[e];
-            const int jb = (int)(p & 0x3FFFFu) - (int)j0;
+          for (int e = 0; e < n; ++e) {
+            const uint32_t p = ep[(int64_t)e * 64];
             const T v = Vs[(p >> 18) & 0xFFFu];
-            s += (((p >> 30) & 1u) ? -v : v) * cb[jb];
+            s += (((p >> 30) & 1u) ? -v : v) * cb[p & 0x3FFFFu];
           }
           acc[m] += coef * s;
         }
@@ -225,8 +217,11 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, const uint64_t* 
   }
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    const int64_t ib = threadIdx.x + (int64_t)m * NT;
-    if (ib < nb) sigma[ia * nb + ib] += acc[m];
+    const int64_t pos = col0 + (int64_t)m * NT + threadIdx.x;
+    if ((pos >> 6) < groups) {
+      const int32_t ib = colperm[pos];
+      if (ib >= 0) sigma[ia * nb + ib] += acc[m];
+    }
   }
 }
 
@@ -261,29 +256,31 @@ void launch_transpose(int64_t R, int64_t C, const T* in, T* out, hipStream_t st)
 
 template <class T, int M>
 void launch_ab_m(const SigmaPlan& P, int nt, size_t lds, const T* eri, const T* c, T* sigma, hipStream_t st) {
-  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)P.na), dim3(nt), lds, st, P.nb, P.L, P.a, P.b,
-                     P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, P.ab->pack.p, P.ab->cptr.p, P.ab->nch,
-                     P.ab->chunk, eri, c, sigma);
+  const int64_t ntiles = (P.nb + (int64_t)nt * M - 1) / ((int64_t)nt * M);
+  const int64_t nblk = ntiles * P.na;
+  if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
+  const EllTable& E = *P.ell;
+  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(nt), lds, st, P.nb, P.L, ntiles, P.a,
+                     P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.off.p, E.K.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
+                     E.groups, eri, c, sigma);
 }
 
 template <class T>
 void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream_t st) {
   if (P.na == 0 || P.nb == 0) return;
-  int nt = (int)std::min<int64_t>(1024, ((P.nb + 63) / 64) * 64);
+  const int nt = (int)std::min<int64_t>(1024, ((P.nb + 63) / 64) * 64);
   const int64_t need = (P.nb + nt - 1) / nt;
-  const size_t lds = ab_lds_bytes(P.L, P.ab->chunk, (int)sizeof(T));
+  const size_t lds = ab_lds_bytes(P.L, P.ell->chunk, (int)sizeof(T));
+  // one tile per row when the accumulators fit registers (the ket row is then staged once
+  // per pair); wider rows take several tiles of 24 columns per thread
   if (need <= 1) launch_ab_m<T, 1>(P, nt, lds, eri, c, sigma, st);
   else if (need <= 2) launch_ab_m<T, 2>(P, nt, lds, eri, c, sigma, st);
   else if (need <= 4) launch_ab_m<T, 4>(P, nt, lds, eri, c, sigma, st);
   else if (need <= 8) launch_ab_m<T, 8>(P, nt, lds, eri, c, sigma, st);
   else if (need <= 12) launch_ab_m<T, 12>(P, nt, lds, eri, c, sigma, st);
   else if (need <= 16) launch_ab_m<T, 16>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 24) launch_ab_m<T, 24>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 32) launch_ab_m<T, 32>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 40) launch_ab_m<T, 40>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 48) launch_ab_m<T, 48>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 64) launch_ab_m<T, 64>(P, nt, lds, eri, c, sigma, st);
-  else throw Error(SBDG_ERR_LIMIT, "beta string set beyond 65536 strings");
+  else if (need <= 20) launch_ab_m<T, 20>(P, nt, lds, eri, c, sigma, st);
+  else launch_ab_m<T, 24>(P, nt, lds, eri, c, sigma, st);
 }
 
 }  // namespace
@@ -294,7 +291,7 @@ void transpose_d(int64_t R, int64_t C, const double* in, double* out, hipStream_
 }
 
 size_t ab_lds_bytes(int L, int64_t chunk, int elem) {
-  return (size_t)elem * ((size_t)chunk + (size_t)L * L);
+  return (size_t)elem * ((size_t)chunk + 1 + (size_t)L * L);
 }
 
 template <class T>

@@ -240,6 +240,87 @@ __global__ void k_abchunk(int64_t n, int nch, int64_t chunk, const int64_t* __re
   cptr[t] = (int32_t)lo;
 }
 
+__device__ inline int64_t lower_bound_col(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int64_t v) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)col[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// cnt[ib*nch+k]: entries of beta row ib with ket in chunk k, plus the occupied orbitals of
+// ib when ib itself lies in chunk k (pseudo-entries)
+__global__ void k_ell_count(int64_t npos, int nch, int64_t chunk, const int32_t* __restrict__ perm,
+                            const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const uint64_t* __restrict__ bstr, int32_t* __restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npos * nch) return;
+  const int k = (int)(t % nch);
+  const int64_t ib = perm[t / nch];
+  if (ib < 0) {
+    cnt[t] = 0;
+    return;
+  }
+  const int64_t lo = lower_bound_col(col, rp[ib], rp[ib + 1], (int64_t)k * chunk);
+  const int64_t hi = lower_bound_col(col, lo, rp[ib + 1], (int64_t)(k + 1) * chunk);
+  int n = (int)(hi - lo);
+  if (ib / chunk == k) n += __popcll(bstr[ib]);
+  cnt[t] = n;
+}
+
+// K[g*nch+k] = max over the 64 lane positions of group g
+__global__ void k_ell_max(int64_t groups, int nch, const int32_t* __restrict__ cnt,
+                          int64_t* __restrict__ K64, int32_t* __restrict__ K) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= groups * nch) return;
+  const int64_t g = t / nch;
+  const int k = (int)(t % nch);
+  int m = 0;
+  for (int l = 0; l < 64; ++l) m = max(m, cnt[(g * 64 + l) * nch + k]);
+  K[t] = m;
+  K64[t] = (int64_t)m * 64;
+}
+
+__global__ void k_ell_fill(int64_t groups, int nch, int64_t chunk, int L, const int32_t* __restrict__ perm,
+                           const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                           const uint32_t* __restrict__ exc, const uint64_t* __restrict__ bstr,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ K,
+                           uint32_t* __restrict__ ent) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= groups * 64 * nch) return;
+  const int64_t pos = t / nch;
+  const int k = (int)(t % nch);
+  const int64_t g = pos >> 6;
+  const int lane = (int)(pos & 63);
+  const int64_t ib = perm[pos];
+  const int64_t idx = g * nch + k;
+  const int64_t base = off[idx] + lane;
+  const int kk = K[idx];
+  const uint32_t pad = (uint32_t)chunk;
+  int e = 0;
+  if (ib >= 0) {
+    const int64_t j0 = (int64_t)k * chunk;
+    const int64_t lo = lower_bound_col(col, rp[ib], rp[ib + 1], j0);
+    const int64_t hi = lower_bound_col(col, lo, rp[ib + 1], j0 + chunk);
+    for (int64_t q = lo; q < hi; ++q, ++e) {
+      const uint32_t x = exc[q];
+      const uint32_t j = x & 63u, b = (x >> 6) & 63u, neg = (x >> 24) & 1u;
+      ent[base + (int64_t)e * 64] = (uint32_t)(col[q] - j0) | ((b * (uint32_t)L + j) << 18) | (neg << 30);
+    }
+    if (ib / chunk == k) {
+      uint64_t y = bstr[ib];
+      while (y) {
+        const uint32_t o = (uint32_t)__builtin_ctzll(y);
+        y &= y - 1;
+        ent[base + (int64_t)e * 64] = (uint32_t)(ib - j0) | ((o * (uint32_t)L + o) << 18);
+        ++e;
+      }
+    }
+  }
+  for (; e < kk; ++e) ent[base + (int64_t)e * 64] = pad;
+}
+
 // per-string diagonal parts: E[s] = sum_{i in s} h_ii + sum_{i<j in s} [(ii|jj)-(ij|ji)]
 __global__ void k_strdiag(const uint64_t* __restrict__ s, int64_t n, int L,
                           const double* __restrict__ h1, const double* __restrict__ eri,
@@ -369,6 +450,46 @@ void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st
     hipLaunchKernelGGL(k_abchunk, dim3(blocks_for(nt, 256)), dim3(256), 0, st, sb.n, ab.nch, chunk,
                        sb.rowptr.p, sb.col.p, ab.cptr.p);
   SBDG_CHECK(hipGetLastError());
+}
+
+void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, EllTable& e, hipStream_t st) {
+  const int64_t nb = sb.n;
+  if (chunk >= (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta chunk beyond 2^18 columns");
+  e.chunk = chunk;
+  e.nch = (int)std::max<int64_t>(1, (nb + chunk - 1) / chunk);
+  e.groups = (nb + 63) / 64;
+  const int64_t npos = e.groups * 64, ng = e.groups * e.nch;
+  // lane positions: columns by decreasing number of beta singles (ties by index)
+  std::vector<int64_t> rp(nb + 1);
+  SBDG_CHECK(hipMemcpy(rp.data(), sb.rowptr.p, sizeof(int64_t) * (nb + 1), hipMemcpyDeviceToHost));
+  std::vector<int32_t> perm(std::max<int64_t>(npos, 1), -1);
+  for (int64_t i = 0; i < nb; ++i) perm[i] = (int32_t)i;
+  std::stable_sort(perm.begin(), perm.begin() + nb,
+                   [&](int32_t x, int32_t y) { return rp[x + 1] - rp[x] > rp[y + 1] - rp[y]; });
+  e.colperm.alloc(std::max<int64_t>(npos, 1));
+  SBDG_CHECK(hipMemcpy(e.colperm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
+  DBuf<int32_t> cnt;
+  DBuf<int64_t> K64;
+  cnt.alloc(std::max<int64_t>(npos * e.nch, 1));
+  K64.alloc(std::max<int64_t>(ng, 1));
+  e.K.alloc(std::max<int64_t>(ng, 1));
+  e.off.alloc(ng + 1);
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_ell_count, dim3(blocks_for(npos * e.nch, 256)), dim3(256), 0, st, npos, e.nch, chunk,
+                       e.colperm.p, sb.rowptr.p, sb.col.p, d_b, cnt.p);
+    hipLaunchKernelGGL(k_ell_max, dim3(blocks_for(ng, 256)), dim3(256), 0, st, e.groups, e.nch, cnt.p, K64.p, e.K.p);
+  }
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, K64.p, ng, e.off.p);
+  SBDG_CHECK(hipGetLastError());
+  int64_t total = 0;
+  SBDG_CHECK(hipMemcpyAsync(&total, e.off.p + ng, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SBDG_CHECK(hipStreamSynchronize(st));
+  e.ent.alloc(std::max<int64_t>(total, 1));
+  if (ng > 0)
+    hipLaunchKernelGGL(k_ell_fill, dim3(blocks_for(ng * 64, 256)), dim3(256), 0, st, e.groups, e.nch, chunk, L,
+                       e.colperm.p, sb.rowptr.p, sb.col.p, sb.exc.p, d_b, e.off.p, e.K.p, e.ent.p);
+  SBDG_CHECK(hipGetLastError());
+  SBDG_CHECK(hipStreamSynchronize(st));
 }
 
 void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
