@@ -29,7 +29,7 @@ using namespace sbdg;
 
 namespace {
 constexpr int MAXSUB = 48;
-constexpr size_t LDS_BUDGET = 80 * 1024;
+constexpr size_t LDS_BUDGET = 144 * 1024;  // alpha-beta kernel, one 1024-thread workgroup per CU
 }  // namespace
 
 struct sbdg_ctx {
@@ -89,8 +89,9 @@ double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// column chunk of the alpha-beta kernel: two ket-row chunks and two L*L slices in LDS
 int64_t choose_chunk(int64_t nb, int L, int elem) {
-  const int64_t budget = (int64_t)(LDS_BUDGET / elem) - (int64_t)L * L - 1;
+  const int64_t budget = (int64_t)(LDS_BUDGET / elem) / 2 - (int64_t)L * L - 1;
   int64_t ch = std::max<int64_t>(256, budget);
   return std::min<int64_t>(std::max<int64_t>(nb, 1), ch);
 }
@@ -134,7 +135,8 @@ void build(sbdg_ctx* c) {
   build_table(c->d_b.p, nb, L, c->d_h1.p, c->d_eri.p, c->Sb, c->Db, c->st);
   build_spmm(c->Sa, c->Da, c->Ma, c->st);
   build_spmm(c->Sb, c->Db, c->Mb, c->st);
-  build_ab(c->Sb, L, choose_chunk(nb, L, 8), c->ab64, c->st);
+  build_ab(c->Sb, L, std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - (int64_t)L * L)),
+           c->ab64, c->st);
   build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 8), c->ell64, c->st);
   build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 4), c->ell32, c->st);
   c->d_hii.alloc(std::max<int64_t>(N, 1));
@@ -270,6 +272,13 @@ void davidson(sbdg_ctx* c) {
     SBDG_CHECK(hipStreamSynchronize(c->st));
   };
 
+  {
+    const sbdg_stats keep_st = c->stats;
+    c->stats = sbdg_stats{};
+    c->stats.t_tables = keep_st.t_tables;
+    c->stats.terms = keep_st.terms;
+    for (int k = 0; k < 4; ++k) c->stats.nnz[k] = keep_st.nnz[k];
+  }
   initial_vector(c, c->V.p);
   h2d(one_dev, 1.0);
   const double tstart = now();

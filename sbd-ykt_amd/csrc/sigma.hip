@@ -133,10 +133,33 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
 
 // alpha-beta and cross-spin single terms.  One workgroup per (bra row ia, tile of lane
 // positions); thread t owns positions col0 + t + m*blockDim.x, m < M (every wave owns
-// whole 64-lane groups of the ELL table), i.e. columns colperm[position].  For each ket row ja in {ia} + S(ia) the (ai|..) slice (or, for
-// ia itself, sum_{k in occ(ia)} (kk|..) with a zeroed diagonal) sits in LDS and the ket
-// row c[ja,:] streams through LDS chunk by chunk; entries are read coalesced from the ELL
-// table with wave-uniform trip counts.  LDS: [chunk + 1 (zero slot)] [L*L slice].
+// whole 64-lane groups of the ELL table), i.e. columns colperm[position].  The ket rows
+// ja in {ia} + S(ia) are taken two at a time: both (ai|..) slices (for ia itself
+// sum_{k in occ(ia)} (kk|..) with a zeroed diagonal) and both rows' current column chunk
+// sit in LDS, and every ELL entry read from the (L2-resident) table serves both rows.
+// LDS: [2 x (chunk + 1 zero slot)] [2 x L*L slices].
+template <class T>
+__device__ inline void ab_slice(bool self, uint64_t A, int ai, int L, int64_t LL, const T* __restrict__ eri,
+                                T* __restrict__ V) {
+  if (self) {
+    for (int x = threadIdx.x; x < LL; x += blockDim.x) {
+      T v = T(0);
+      if (x / L != x % L) {  // the diagonal (kk|jj) part is in hii already
+        uint64_t y = A;
+        while (y) {
+          const int k = __builtin_ctzll(y);
+          y &= y - 1;
+          v += eri[((int64_t)k * L + k) * LL + x];
+        }
+      }
+      V[x] = v;
+    }
+  } else {
+    const T* src = eri + (int64_t)ai * LL;
+    for (int x = threadIdx.x; x < LL; x += blockDim.x) V[x] = src[x];
+  }
+}
+
 template <class T, int M>
 __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, const uint64_t* __restrict__ astr,
                                              const int64_t* __restrict__ rpSa,
@@ -148,54 +171,54 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
                                              int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
                                              T* __restrict__ sigma) {
   extern __shared__ __align__(16) unsigned char smem[];
-  T* cb = reinterpret_cast<T*>(smem);
-  T* Vs = cb + chunk + 1;
+  T* cb0 = reinterpret_cast<T*>(smem);
+  T* cb1 = cb0 + chunk + 1;
+  T* V0 = cb1 + chunk + 1;
+  const int64_t LL = (int64_t)L * L;
+  T* V1 = V0 + LL;
   const int NT = blockDim.x;
   const int64_t ia = blockIdx.x / ntiles;
   const int64_t col0 = (blockIdx.x - ia * ntiles) * (int64_t)NT * M;
   const int lane = threadIdx.x & 63;
-  const int64_t LL = (int64_t)L * L;
-  if (threadIdx.x == 0) cb[chunk] = T(0);
+  if (threadIdx.x == 0) {
+    cb0[chunk] = T(0);
+    cb1[chunk] = T(0);
+  }
   T acc[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = T(0);
   const int64_t q0 = rpSa[ia], q1 = rpSa[ia + 1];
-  for (int64_t t = q0 - 1; t < q1; ++t) {
+  const uint64_t A = astr[ia];
+  for (int64_t t = q0 - 1; t < q1; t += 2) {
     const bool self = (t < q0);
-    int64_t ja = ia;
-    T coef = T(1);
-    int ai = 0;
+    const bool two = (t + 1 < q1);
+    int64_t ja0 = ia, ja1 = 0;
+    T coef0 = T(1), coef1 = T(0);
+    int ai0 = 0, ai1 = 0;
     if (!self) {
-      ja = colSa[t];
+      ja0 = colSa[t];
       const uint32_t e = excSa[t];
-      ai = (int)((e >> 6) & 63u) * L + (int)(e & 63u);  // a*L + i
-      coef = ((e >> 24) & 1u) ? T(-1) : T(1);
+      ai0 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);  // a*L + i
+      coef0 = ((e >> 24) & 1u) ? T(-1) : T(1);
+    }
+    if (two) {
+      ja1 = colSa[t + 1];
+      const uint32_t e = excSa[t + 1];
+      ai1 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);
+      coef1 = ((e >> 24) & 1u) ? T(-1) : T(1);
     }
     __syncthreads();
-    if (self) {
-      const uint64_t A = astr[ia];
-      for (int x = threadIdx.x; x < LL; x += NT) {
-        T v = T(0);
-        if (x / L != x % L) {  // the diagonal (kk|jj) part is in hii already
-          uint64_t y = A;
-          while (y) {
-            const int k = __builtin_ctzll(y);
-            y &= y - 1;
-            v += eri[((int64_t)k * L + k) * LL + x];
-          }
-        }
-        Vs[x] = v;
-      }
-    } else {
-      const T* src = eri + (int64_t)ai * LL;
-      for (int x = threadIdx.x; x < LL; x += NT) Vs[x] = src[x];
-    }
-    const T* crow = c + ja * nb;
+    ab_slice<T>(self, A, ai0, L, LL, eri, V0);
+    if (two) ab_slice<T>(false, A, ai1, L, LL, eri, V1);
+    const T* crow0 = c + ja0 * nb;
+    const T* crow1 = c + ja1 * nb;
     for (int k = 0; k < nch; ++k) {
       const int64_t j0 = (int64_t)k * chunk;
       const int64_t nj = min(nb, j0 + chunk) - j0;
       __syncthreads();
-      for (int64_t x = threadIdx.x; x < nj; x += NT) cb[x] = crow[j0 + x];
+      for (int64_t x = threadIdx.x; x < nj; x += NT) cb0[x] = __builtin_nontemporal_load(crow0 + j0 + x);
+      if (two)
+        for (int64_t x = threadIdx.x; x < nj; x += NT) cb1[x] = __builtin_nontemporal_load(crow1 + j0 + x);
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < M; ++m) {
@@ -204,13 +227,30 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
           const int64_t idx = g * nch + k;
           const int n = eK[idx];
           const uint32_t* ep = ent + eoff[idx] + lane;
-          T s = T(0);
-          for (int e = 0; e < n; ++e) {
-            const uint32_t p = ep[(int64_t)e * 64];
-            const T v = Vs[(p >> 18) & 0xFFFu];
-            s += (((p >> 30) & 1u) ? -v : v) * cb[p & 0x3FFFFu];
+          if (two) {
+            T s0 = T(0), s1 = T(0);
+            for (int e = 0; e < n; ++e) {
+              const uint32_t p = ep[(int64_t)e * 64];
+              const uint32_t pi = (p >> 18) & 0xFFFu, jb = p & 0x3FFFFu;
+              const T x0 = V0[pi] * cb0[jb], x1 = V1[pi] * cb1[jb];
+              if ((p >> 30) & 1u) {
+                s0 -= x0;
+                s1 -= x1;
+              } else {
+                s0 += x0;
+                s1 += x1;
+              }
+            }
+            acc[m] += coef0 * s0 + coef1 * s1;
+          } else {
+            T s0 = T(0);
+            for (int e = 0; e < n; ++e) {
+              const uint32_t p = ep[(int64_t)e * 64];
+              const T v = V0[(p >> 18) & 0xFFFu];
+              s0 += (((p >> 30) & 1u) ? -v : v) * cb0[p & 0x3FFFFu];
+            }
+            acc[m] += coef0 * s0;
           }
-          acc[m] += coef * s;
         }
       }
     }
@@ -291,7 +331,7 @@ void transpose_d(int64_t R, int64_t C, const double* in, double* out, hipStream_
 }
 
 size_t ab_lds_bytes(int L, int64_t chunk, int elem) {
-  return (size_t)elem * ((size_t)chunk + 1 + (size_t)L * L);
+  return (size_t)elem * 2 * ((size_t)chunk + 1 + (size_t)L * L);
 }
 
 template <class T>
