@@ -13,9 +13,12 @@
 // Mixed precision (options.precision = 1): basis vectors, Gram-Schmidt, the subspace
 // eigensolve and all residual bookkeeping stay FP64; the sigma of each new basis vector
 // is built in FP32 (FP32 vectors, integrals and per-row accumulation).  Every refresh_k
-// steps the current Ritz vector x becomes the first basis vector and its sigma is rebuilt
-// in FP64, which gives the exact Rayleigh quotient and residual (used for convergence
-// and reported) and an exact first row/column of the projected matrix.
+// steps the basis is rotated onto the Ritz vectors of the subspace (no vector dropped;
+// a full basis restarts to restart_keep as in the FP64 path), so the current Ritz
+// vector x is basis vector 0, and its sigma is rebuilt in FP64.  That gives the exact
+// Rayleigh quotient and residual of x (the only quantities used for convergence and
+// reported), an exact first row/column of the projected matrix, and the exact residual
+// drives the next correction vector.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -419,13 +422,15 @@ void davidson(sbdg_ctx* c) {
         expand_or_stop();
         continue;
       }
-      // the Ritz vector becomes basis vector 0 (thick restart to `keep` vectors)
-      rotate_impl<double>(c->V.p, N, h, s.U, MAXSUB, keep, N, c->st);
-      rotate_impl<float>(c->HV32.p, N, h, s.U, MAXSUB, keep, N, c->st);
+      // the basis becomes the Ritz vectors of the subspace (all of them, or `keep` when
+      // the basis is full: thick restart), so the Ritz vector x is basis vector 0
+      const int kk = (v == nbmax) ? keep : h;
+      rotate_impl<double>(c->V.p, N, h, s.U, MAXSUB, kk, N, c->st);
+      rotate_impl<float>(c->HV32.p, N, h, s.U, MAXSUB, kk, N, c->st);
       SBDG_CHECK(hipMemcpyAsync(c->V.p, c->W.p, sizeof(double) * N, hipMemcpyDeviceToDevice, c->st));
       std::fill(Hs.begin(), Hs.end(), 0.0);
-      for (int k = 1; k < keep; ++k) Hs[k + MAXSUB * k] = E[k];
-      v = h = keep;
+      for (int k = 1; k < kk; ++k) Hs[k + MAXSUB * k] = E[k];
+      v = h = kk;
       ++cycle;
       refresh();
       have_anchor = true;
