@@ -54,6 +54,9 @@ typedef struct {
   double target_de;   /* mixed: also stop when |E - E_prev refresh| < target_de (0: off) */
   int verbose;        /* 1: one line per Davidson step on stdout */
   uint64_t seed;      /* init==1: random start vector seed */
+  int doubles;        /* same-spin doubles in sigma: 0 auto, 1 excitation tables (as
+                         mult), 2 grouped by their (n-2)-electron core */
+  double core_gib;    /* doubles==2: workspace cap in GiB for the per-core products */
 } sbdg_options;
 
 typedef struct {

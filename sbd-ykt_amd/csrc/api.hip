@@ -52,6 +52,11 @@ struct sbdg_ctx {
   DBuf<uint64_t> d_a, d_b;
   Table Sa, Da, Sb, Db;
   SpmmTable Ma, Mb;
+  CoreTable Ca, Cb;       // same-spin doubles by core (used when use_ca / use_cb)
+  bool use_ca = false, use_cb = false;
+  DBuf<double> Wp;        // antisymmetrised pair integrals
+  DBuf<float> Wp32;
+  DBuf<double> Zbuf;      // core-product workspace
   AbTable ab64;     // beta singles by chunk (RDMs)
   EllTable ell64, ell32;  // beta singles, ELL (alpha-beta sigma)
   // solver state
@@ -114,6 +119,13 @@ SigmaPlan plan(sbdg_ctx* c, bool fp32) {
   P.eri32 = c->d_eri32.p;
   P.hii = c->d_hii.p;
   P.hii32 = c->d_hii32.p;
+  P.ca = c->use_ca ? &c->Ca : nullptr;
+  P.cb = c->use_cb ? &c->Cb : nullptr;
+  P.W = c->Wp.p;
+  P.W32 = c->Wp32.p;
+  P.npair = (int64_t)c->L * (c->L - 1) / 2;
+  P.Z = c->Zbuf.p;
+  P.zbytes = c->Zbuf.n * sizeof(double);
   return P;
 }
 
@@ -136,8 +148,36 @@ void build(sbdg_ctx* c) {
   SBDG_CHECK(hipMemcpyAsync(c->d_b.p, c->b.data(), sizeof(uint64_t) * nb, hipMemcpyHostToDevice, c->st));
   build_table(c->d_a.p, na, L, c->d_h1.p, c->d_eri.p, c->Sa, c->Da, c->st);
   build_table(c->d_b.p, nb, L, c->d_h1.p, c->d_eri.p, c->Sb, c->Db, c->st);
-  build_spmm(c->Sa, c->Da, c->Ma, c->st);
-  build_spmm(c->Sb, c->Db, c->Mb, c->st);
+  // same-spin doubles: excitation table (as mult) or grouped by core.  Auto picks cores
+  // when the table would stream more than four times the member rows a core pass moves.
+  const int nel_a = na ? __builtin_popcountll(c->a[0]) : 0, nel_b = nb ? __builtin_popcountll(c->b[0]) : 0;
+  c->use_ca = c->use_cb = false;
+  if (c->opt.doubles != 1) {
+    build_cores(c->d_a.p, na, L, nel_a, c->Ca, c->st);
+    build_cores(c->d_b.p, nb, L, nel_b, c->Cb, c->st);
+    c->use_ca = c->opt.doubles == 2 || c->Da.nnz > 4 * c->Ca.nmem;
+    c->use_cb = c->opt.doubles == 2 || c->Db.nnz > 4 * c->Cb.nmem;
+  }
+  build_spmm(c->Sa, c->Da, c->Ma, c->st, !c->use_ca);
+  build_spmm(c->Sb, c->Db, c->Mb, c->st, !c->use_cb);
+  if (c->use_ca || c->use_cb) {
+    const int64_t npair = (int64_t)L * (L - 1) / 2;
+    c->Wp.alloc(std::max<int64_t>(npair * npair, 1));
+    c->Wp32.alloc(std::max<int64_t>(npair * npair, 1));
+    build_pair_integrals(c->d_eri.p, L, c->Wp.p, c->Wp32.p, c->st);
+    auto need = [](const CoreTable& ct, int64_t ncols) {
+      return (size_t)ct.nmem * (size_t)((ncols + CORE_COLS - 1) / CORE_COLS * CORE_COLS) * sizeof(double);
+    };
+    auto least = [](const CoreTable& ct) { return (size_t)ct.nmem * CORE_COLS * sizeof(double); };
+    size_t want = 0, low = 0;
+    if (c->use_ca) want = std::max(want, need(c->Ca, nb)), low = std::max(low, least(c->Ca));
+    if (c->use_cb) want = std::max(want, need(c->Cb, na)), low = std::max(low, least(c->Cb));
+    const size_t cap = (size_t)(c->opt.core_gib * 1073741824.0);
+    if (low > cap) throw Error(SBDG_ERR_LIMIT, "core_gib below one 512-column pass of the core products");
+    c->Zbuf.alloc(std::max<size_t>(std::min(want, cap) / sizeof(double), 1));
+  } else {
+    c->Zbuf.free();
+  }
   build_ab(c->Sb, L, std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - (int64_t)L * L)),
            c->ab64, c->st);
   build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 8), c->ell64, c->st);
@@ -542,6 +582,8 @@ int sbdg_default_options(sbdg_options* o) {
   o->target_de = 0.0;
   o->verbose = 0;
   o->seed = 12345;
+  o->doubles = 0;
+  o->core_gib = 8.0;
   return SBDG_OK;
 }
 
@@ -620,6 +662,9 @@ int sbdg_set_options(sbdg_ctx* c, const sbdg_options* o) {
     if (o->precision != 0 && o->precision != 1) return fail(c, SBDG_ERR_ARG, "precision must be 0 or 1");
     if (o->init < 0 || o->init > 2) return fail(c, SBDG_ERR_ARG, "init must be 0, 1 or 2");
     if (!(o->eps >= 0.0)) return fail(c, SBDG_ERR_ARG, "eps must be >= 0");
+    if (o->doubles < 0 || o->doubles > 2) return fail(c, SBDG_ERR_ARG, "doubles must be 0, 1 or 2");
+    if (!(o->core_gib > 0.0)) return fail(c, SBDG_ERR_ARG, "core_gib must be > 0");
+    if (c->built && (o->doubles != c->opt.doubles || o->core_gib != c->opt.core_gib)) c->built = false;
     c->opt = *o;
     return SBDG_OK;
   });

@@ -93,9 +93,9 @@ struct AbTable {
 };
 
 // Beta singles in an ELL layout for the alpha-beta sigma kernel.  For every group g of 64
-// consecutive bra columns and column chunk k there are K[g*nch+k] entries per lane, stored
-// lane-interleaved at off[g*nch+k] + e*64 + lane and padded with a zero-weight entry, so a
-// wave reads them coalesced with a wave-uniform trip count.  Entry: bits 0-17 ket column
+// consecutive bra columns and column chunk k there are K[g*nch+k] entries per lane (a
+// multiple of 4), stored lane-interleaved at off[g*nch+k] + e*64 + lane and padded with a
+// zero-weight entry, so a wave reads them coalesced with a wave-uniform trip count.  Entry: bits 0-17 ket column
 // offset inside the chunk (the value `chunk` addresses a zero slot: padding), bits 18-29
 // orbital pair b*L+j, bit 30 sign.  The beta-occupation part of the alpha singles rides
 // along as pseudo-entries: ket column = bra column, pair k*L+k for each occupied k.
@@ -111,10 +111,36 @@ struct EllTable {
   DBuf<int32_t> colperm;  // groups*64 (positions beyond nb: -1)
 };
 
+// Double excitations of one spin grouped by their (n-2)-electron core: a double pair
+// (x, y) shares exactly one core k = x & y (two electrons fewer than x), and its element
+// in the separated order is eps_x * eps_y * W[P_x][P_y], with P_x = (a < b) the pair added
+// to k to make x, eps_x = <x| c+_a c+_b |k> and W[(A,B)][(I,J)] = (AI|BJ) - (AJ|BI)
+// (zero when the pairs overlap).  Members are stored core by core (only cores with at
+// least two members); row slots list, per string, the member entries it owns.
+struct CoreTable {
+  int64_t ncore = 0, nmem = 0, nblk = 0;
+  int maxm = 0;
+  DBuf<int32_t> start;    // ncore + 1
+  DBuf<int32_t> row;      // nmem
+  DBuf<int32_t> pair;     // nmem: P(a,b) = b(b-1)/2 + a, bit 31 set when eps is negative
+  DBuf<int32_t> blk;      // nblk: core of each product block
+  DBuf<int32_t> blk_x0;   // nblk: first member of the block
+  DBuf<int32_t> slot_ptr; // n + 1
+  DBuf<int32_t> slot;     // nmem
+  int64_t ngrp = 0;
+  DBuf<int32_t> grp;      // ngrp + 1: product blocks of each workgroup
+};
+constexpr int CORE_XB = 8;      // output members per product block
+constexpr int CORE_COLS = 512;  // column granularity of the core-product passes
+constexpr int CORE_WORK = 256;  // member rows streamed per core-product workgroup (target)
+
 // kernels (tables.hip)
 void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, const double* d_eri,
                  Table& t_single, Table& t_double, hipStream_t st);
-void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st);
+void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st, bool with_doubles = true);
+void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct, hipStream_t st);
+// W[P*npair + Q] (FP64 and FP32), npair = L(L-1)/2
+void build_pair_integrals(const double* d_eri, int L, double* W, float* W32, hipStream_t st);
 void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st);
 void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, EllTable& e, hipStream_t st);
 void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
@@ -134,6 +160,13 @@ struct SigmaPlan {
   const float* eri32 = nullptr;
   const double* hii = nullptr;
   const float* hii32 = nullptr;
+  const CoreTable* ca = nullptr;  // alpha doubles by core (null: doubles in ma/mb)
+  const CoreTable* cb = nullptr;
+  const double* W = nullptr;
+  const float* W32 = nullptr;
+  int64_t npair = 0;
+  void* Z = nullptr;    // core-product workspace of zbytes
+  size_t zbytes = 0;
 };
 // sigma = H c in the separated convention; scratch holds 2*na*nb elements of T.
 template <class T>

@@ -35,21 +35,26 @@ struct V16<float> {
 
 constexpr int TS = 256;  // threads of the row-streaming product
 
-// out[r, tile] = (DIAG ? hii*X : 0)[r, tile] + sum_q val[q] X[col[q], tile]
-template <class T, bool DIAG, bool VECTOR>
-__global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, const int64_t* __restrict__ rowptr,
-                                             const int32_t* __restrict__ col, const T* __restrict__ val,
-                                             const T* __restrict__ X, const T* __restrict__ hii,
-                                             T* __restrict__ out) {
+// Column range [cb, ce) of one row product:
+//   out[r, c] = (DIAG ? hii*X : 0)[r, c] + sum_q val[q] X[col[q], c]
+//             + (ZG ? sum_{s in slots(r)} Z[slot[s], c - cb] : 0)
+// Z holds the core-grouped doubles of the same column range (k_core_prod).
+template <class T, bool DIAG, bool VECTOR, bool ZG>
+__global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64_t cb, int64_t ce,
+                                             const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                             const T* __restrict__ val, const T* __restrict__ X,
+                                             const T* __restrict__ hii, T* __restrict__ out,
+                                             const int32_t* __restrict__ zptr, const int32_t* __restrict__ zslot,
+                                             const T* __restrict__ Z, int64_t zcols) {
   constexpr int VN = V16<T>::n;
   using VT = typename V16<T>::t;
   const int64_t bid = blockIdx.x;
   const int64_t tile = bid / nrows;
   const int64_t r = bid - tile * nrows;
-  const int64_t c0 = tile * (int64_t)(TS * VN) + (int64_t)threadIdx.x * VN;
-  if (c0 >= ncols) return;
+  const int64_t c0 = cb + tile * (int64_t)(TS * VN) + (int64_t)threadIdx.x * VN;
+  if (c0 >= ce) return;
   T acc[VN];
-  const bool full = VECTOR && (c0 + VN <= ncols);
+  const bool full = VECTOR && (c0 + VN <= ce);
   if (DIAG) {
     const T* h = hii + r * ncols + c0;
     const T* x = X + r * ncols + c0;
@@ -61,7 +66,7 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, const
       for (int k = 0; k < VN; ++k) acc[k] = hv[k] * xv[k];
     } else {
 #pragma unroll
-      for (int k = 0; k < VN; ++k) acc[k] = (c0 + k < ncols) ? h[k] * x[k] : T(0);
+      for (int k = 0; k < VN; ++k) acc[k] = (c0 + k < ce) ? h[k] * x[k] : T(0);
     }
   } else {
 #pragma unroll
@@ -93,6 +98,24 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, const
 #pragma unroll
       for (int k = 0; k < VN; ++k) acc[k] += v * x[k];
     }
+    if (ZG) {
+      const int64_t zc = c0 - cb;
+      int32_t s = zptr[r];
+      const int32_t s1 = zptr[r + 1];
+      for (; s + 2 <= s1; s += 2) {
+        T z0[VN], z1[VN];
+        *reinterpret_cast<VT*>(z0) = *reinterpret_cast<const VT*>(Z + (int64_t)zslot[s] * zcols + zc);
+        *reinterpret_cast<VT*>(z1) = *reinterpret_cast<const VT*>(Z + (int64_t)zslot[s + 1] * zcols + zc);
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc[k] += z0[k] + z1[k];
+      }
+      if (s < s1) {
+        T z0[VN];
+        *reinterpret_cast<VT*>(z0) = *reinterpret_cast<const VT*>(Z + (int64_t)zslot[s] * zcols + zc);
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc[k] += z0[k];
+      }
+    }
     *reinterpret_cast<VT*>(out + r * ncols + c0) = *reinterpret_cast<VT*>(acc);
   } else {
     for (; q < q1; ++q) {
@@ -100,11 +123,101 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, const
       const T* x = X + (int64_t)col[q] * ncols + c0;
 #pragma unroll
       for (int k = 0; k < VN; ++k)
-        if (c0 + k < ncols) acc[k] += v * x[k];
+        if (c0 + k < ce) acc[k] += v * x[k];
+    }
+    if (ZG) {
+      for (int32_t s = zptr[r]; s < zptr[r + 1]; ++s) {
+        const T* z = Z + (int64_t)zslot[s] * zcols + (c0 - cb);
+#pragma unroll
+        for (int k = 0; k < VN; ++k)
+          if (c0 + k < ce) acc[k] += z[k];
+      }
     }
 #pragma unroll
     for (int k = 0; k < VN; ++k)
-      if (c0 + k < ncols) out[r * ncols + c0 + k] = acc[k];
+      if (c0 + k < ce) out[r * ncols + c0 + k] = acc[k];
+  }
+}
+
+// Same-spin doubles grouped by core, column range [cb, ce):
+//   Z[x, c - cb] = sum_{y in core(x)} eps_x eps_y W[P_x][P_y] X[row_y, c]
+// for the members x of one product block (up to CORE_XB members of one core).  Members
+// whose pairs overlap have W = 0 (they differ by a single excitation, or are equal), so
+// every double pair of the string set is counted exactly once, in the core it shares.
+// The block's coefficients are wave-uniform (scalar loads); each lane owns VN columns.
+// workgroup = (group, column tile), tiles fastest (consecutive workgroups = consecutive tiles, so a
+// tile stays on one XCD for every group when the tile count is a multiple of 8), then
+// block groups.
+template <class T, bool VECTOR>
+__global__ void __launch_bounds__(TS) k_core_prod(int64_t ncols, int64_t cb, int64_t ce,
+                                                  const int32_t* __restrict__ start,
+                                                  const int32_t* __restrict__ row,
+                                                  const int32_t* __restrict__ pair,
+                                                  const int32_t* __restrict__ blk,
+                                                  const int32_t* __restrict__ blk_x0,
+                                                  const int32_t* __restrict__ grp, const T* __restrict__ W,
+                                                  int64_t npair, const T* __restrict__ X, T* __restrict__ Z,
+                                                  int64_t zcols) {
+  constexpr int VN = V16<T>::n;
+  constexpr int XB = CORE_XB;
+  using VT = typename V16<T>::t;
+  const int64_t ntiles = (ce - cb + TS * VN - 1) / (TS * VN);
+  const int64_t g = blockIdx.x / ntiles;
+  const int64_t c0 = cb + ((int64_t)blockIdx.x - g * ntiles) * (TS * VN) + (int64_t)threadIdx.x * VN;
+  if (c0 >= ce) return;
+  const bool full = VECTOR && (c0 + VN <= ce);
+  const int64_t zc = c0 - cb;
+  for (int b = grp[g]; b < grp[g + 1]; ++b) {
+    const int core = blk[b];
+    const int s0 = start[core];
+    const int m = start[core + 1] - s0;
+    const int x0 = blk_x0[b];
+    const int nx = min(XB, m - x0);
+    const T* wr[XB];
+    bool nxg[XB];
+#pragma unroll
+    for (int xi = 0; xi < XB; ++xi) {
+      const int pr = xi < nx ? pair[s0 + x0 + xi] : 0;
+      wr[xi] = W + (int64_t)(pr & 0x7FFFFFFF) * npair;
+      nxg[xi] = pr < 0;
+    }
+    T acc[XB][VN];
+#pragma unroll
+    for (int xi = 0; xi < XB; ++xi)
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc[xi][k] = T(0);
+    for (int y = 0; y < m; ++y) {
+      const int pr = pair[s0 + y];
+      const int py = pr & 0x7FFFFFFF;
+      const bool ny = pr < 0;
+      const T* x = X + (int64_t)row[s0 + y] * ncols + c0;
+      T xv[VN];
+      if (full) {
+        *reinterpret_cast<VT*>(xv) = *reinterpret_cast<const VT*>(x);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VN; ++k) xv[k] = (c0 + k < ce) ? x[k] : T(0);
+      }
+#pragma unroll
+      for (int xi = 0; xi < XB; ++xi) {
+        const T w = (nxg[xi] != ny) ? -wr[xi][py] : wr[xi][py];
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc[xi][k] += w * xv[k];
+      }
+    }
+#pragma unroll
+    for (int xi = 0; xi < XB; ++xi) {
+      if (xi < nx) {
+        T* z = Z + (int64_t)(s0 + x0 + xi) * zcols + zc;
+        if (full) {
+          *reinterpret_cast<VT*>(z) = *reinterpret_cast<VT*>(acc[xi]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < VN; ++k)
+            if (c0 + k < ce) z[k] = acc[xi][k];
+        }
+      }
+    }
   }
 }
 
@@ -136,11 +249,30 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
 // whole 64-lane groups of the ELL table), i.e. columns colperm[position].  The ket rows
 // ja in {ia} + S(ia) are taken two at a time: both (ai|..) slices (for ia itself
 // sum_{k in occ(ia)} (kk|..) with a zeroed diagonal) and both rows' current column chunk
-// sit in LDS, and every ELL entry read from the (L2-resident) table serves both rows.
-// LDS: [2 x (chunk + 1 zero slot)] [2 x L*L slices].
+// sit in LDS interleaved (one 2-element LDS read serves both rows), and every ELL entry
+// read from the L2-resident table serves both rows.  Entry lists are padded to multiples
+// of 4 and read four at a time.  LDS: [(chunk + 1 zero slot) pairs] [L*L slice pairs].
+template <class T>
+struct Pair2;
+template <>
+struct Pair2<double> {
+  using t = double2;
+  __device__ static inline double flip(double v, uint32_t neg) {
+    return __longlong_as_double(__double_as_longlong(v) ^ ((unsigned long long)neg << 63));
+  }
+};
+template <>
+struct Pair2<float> {
+  using t = float2;
+  __device__ static inline float flip(float v, uint32_t neg) {
+    return __int_as_float(__float_as_int(v) ^ (int)(neg << 31));
+  }
+};
+
 template <class T>
 __device__ inline void ab_slice(bool self, uint64_t A, int ai, int L, int64_t LL, const T* __restrict__ eri,
-                                T* __restrict__ V) {
+                                T* __restrict__ V, int lane_of_pair) {
+  // V points at the pair array; element x of this row lives at V[2*x + lane_of_pair]
   if (self) {
     for (int x = threadIdx.x; x < LL; x += blockDim.x) {
       T v = T(0);
@@ -152,11 +284,11 @@ __device__ inline void ab_slice(bool self, uint64_t A, int ai, int L, int64_t LL
           v += eri[((int64_t)k * L + k) * LL + x];
         }
       }
-      V[x] = v;
+      V[2 * x + lane_of_pair] = v;
     }
   } else {
     const T* src = eri + (int64_t)ai * LL;
-    for (int x = threadIdx.x; x < LL; x += blockDim.x) V[x] = src[x];
+    for (int x = threadIdx.x; x < LL; x += blockDim.x) V[2 * x + lane_of_pair] = src[x];
   }
 }
 
@@ -170,20 +302,18 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
                                              const int32_t* __restrict__ colperm, int nch, int64_t chunk,
                                              int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
                                              T* __restrict__ sigma) {
+  using T2 = typename Pair2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  T* cb0 = reinterpret_cast<T*>(smem);
-  T* cb1 = cb0 + chunk + 1;
-  T* V0 = cb1 + chunk + 1;
+  T2* cbp = reinterpret_cast<T2*>(smem);  // chunk + 1 pairs
+  T2* Vp = cbp + chunk + 1;               // L*L pairs
+  T* cbs = reinterpret_cast<T*>(cbp);
+  T* Vs = reinterpret_cast<T*>(Vp);
   const int64_t LL = (int64_t)L * L;
-  T* V1 = V0 + LL;
   const int NT = blockDim.x;
   const int64_t ia = blockIdx.x / ntiles;
   const int64_t col0 = (blockIdx.x - ia * ntiles) * (int64_t)NT * M;
   const int lane = threadIdx.x & 63;
-  if (threadIdx.x == 0) {
-    cb0[chunk] = T(0);
-    cb1[chunk] = T(0);
-  }
+  if (threadIdx.x == 0) cbp[chunk] = T2{T(0), T(0)};
   T acc[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = T(0);
@@ -192,7 +322,7 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
   for (int64_t t = q0 - 1; t < q1; t += 2) {
     const bool self = (t < q0);
     const bool two = (t + 1 < q1);
-    int64_t ja0 = ia, ja1 = 0;
+    int64_t ja0 = ia, ja1 = ia;
     T coef0 = T(1), coef1 = T(0);
     int ai0 = 0, ai1 = 0;
     if (!self) {
@@ -208,49 +338,45 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
       coef1 = ((e >> 24) & 1u) ? T(-1) : T(1);
     }
     __syncthreads();
-    ab_slice<T>(self, A, ai0, L, LL, eri, V0);
-    if (two) ab_slice<T>(false, A, ai1, L, LL, eri, V1);
+    ab_slice<T>(self, A, ai0, L, LL, eri, Vs, 0);
+    if (two) ab_slice<T>(false, A, ai1, L, LL, eri, Vs, 1);
+    else
+      for (int x = threadIdx.x; x < LL; x += NT) Vs[2 * x + 1] = T(0);
     const T* crow0 = c + ja0 * nb;
     const T* crow1 = c + ja1 * nb;
     for (int k = 0; k < nch; ++k) {
       const int64_t j0 = (int64_t)k * chunk;
       const int64_t nj = min(nb, j0 + chunk) - j0;
       __syncthreads();
-      for (int64_t x = threadIdx.x; x < nj; x += NT) cb0[x] = __builtin_nontemporal_load(crow0 + j0 + x);
-      if (two)
-        for (int64_t x = threadIdx.x; x < nj; x += NT) cb1[x] = __builtin_nontemporal_load(crow1 + j0 + x);
+      for (int64_t x = threadIdx.x; x < nj; x += NT) {
+        T2 v;
+        v.x = __builtin_nontemporal_load(crow0 + j0 + x);
+        v.y = two ? __builtin_nontemporal_load(crow1 + j0 + x) : T(0);
+        cbp[x] = v;
+      }
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         const int64_t g = (col0 + (int64_t)m * NT + threadIdx.x) >> 6;  // wave-uniform
         if (g < groups) {
           const int64_t idx = g * nch + k;
-          const int n = eK[idx];
+          const int n = eK[idx];  // multiple of 4
           const uint32_t* ep = ent + eoff[idx] + lane;
-          if (two) {
-            T s0 = T(0), s1 = T(0);
-            for (int e = 0; e < n; ++e) {
-              const uint32_t p = ep[(int64_t)e * 64];
-              const uint32_t pi = (p >> 18) & 0xFFFu, jb = p & 0x3FFFFu;
-              const T x0 = V0[pi] * cb0[jb], x1 = V1[pi] * cb1[jb];
-              if ((p >> 30) & 1u) {
-                s0 -= x0;
-                s1 -= x1;
-              } else {
-                s0 += x0;
-                s1 += x1;
-              }
+          T s0 = T(0), s1 = T(0);
+          for (int e = 0; e < n; e += 4) {
+            uint32_t p[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) p[u] = ep[(uint32_t)(e + u) * 64u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const T2 v = Vp[(p[u] >> 18) & 0xFFFu];
+              const T2 x = cbp[p[u] & 0x3FFFFu];
+              const uint32_t neg = (p[u] >> 30) & 1u;
+              s0 += Pair2<T>::flip(v.x, neg) * x.x;
+              s1 += Pair2<T>::flip(v.y, neg) * x.y;
             }
-            acc[m] += coef0 * s0 + coef1 * s1;
-          } else {
-            T s0 = T(0);
-            for (int e = 0; e < n; ++e) {
-              const uint32_t p = ep[(int64_t)e * 64];
-              const T v = V0[(p >> 18) & 0xFFFu];
-              s0 += (((p >> 30) & 1u) ? -v : v) * cb0[p & 0x3FFFFu];
-            }
-            acc[m] += coef0 * s0;
           }
+          acc[m] += coef0 * s0 + coef1 * s1;
         }
       }
     }
@@ -267,23 +393,82 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
 
 template <class T>
 void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, const T* X, const T* hii,
-                 T* out, hipStream_t st) {
-  if (nrows == 0 || ncols == 0) return;
+                 T* out, hipStream_t st, int64_t cb = 0, int64_t ce = -1, const CoreTable* ct = nullptr,
+                 const T* Z = nullptr, int64_t zcols = 0) {
+  if (ce < 0) ce = ncols;
+  if (nrows == 0 || ce <= cb) return;
   constexpr int VN = V16<T>::n;
-  const int64_t ntiles = (ncols + TS * VN - 1) / (TS * VN);
+  const int64_t ntiles = (ce - cb + TS * VN - 1) / (TS * VN);
   const int64_t nblk = ntiles * nrows;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
   const T* val = nullptr;
   if constexpr (sizeof(T) == 8) val = reinterpret_cast<const T*>(tb.val.p);
   else val = reinterpret_cast<const T*>(tb.val32.p);
-  const bool vec = (ncols % VN) == 0;
+  const bool vec = (ncols % VN) == 0 && (cb % VN) == 0;
+  const int32_t* zp = ct ? ct->slot_ptr.p : nullptr;
+  const int32_t* zs = ct ? ct->slot.p : nullptr;
   dim3 g((unsigned)nblk), b(TS);
-  if (diag) {
-    if (vec) hipLaunchKernelGGL((k_spmm<T, true, true>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
-    else hipLaunchKernelGGL((k_spmm<T, true, false>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
+#define SBDG_SPMM(D, V, ZG)                                                                                   \
+  hipLaunchKernelGGL((k_spmm<T, D, V, ZG>), g, b, 0, st, nrows, ncols, cb, ce, tb.rowptr.p, tb.col.p, val, X, \
+                     hii, out, zp, zs, Z, zcols)
+  if (ct) {
+    if (diag) {
+      if (vec) SBDG_SPMM(true, true, true);
+      else SBDG_SPMM(true, false, true);
+    } else {
+      if (vec) SBDG_SPMM(false, true, true);
+      else SBDG_SPMM(false, false, true);
+    }
   } else {
-    if (vec) hipLaunchKernelGGL((k_spmm<T, false, true>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
-    else hipLaunchKernelGGL((k_spmm<T, false, false>), g, b, 0, st, nrows, ncols, tb.rowptr.p, tb.col.p, val, X, hii, out);
+    if (diag) {
+      if (vec) SBDG_SPMM(true, true, false);
+      else SBDG_SPMM(true, false, false);
+    } else {
+      if (vec) SBDG_SPMM(false, true, false);
+      else SBDG_SPMM(false, false, false);
+    }
+  }
+#undef SBDG_SPMM
+}
+
+// Z columns per pass over the range of one spin: a multiple of CORE_COLS that fits the
+// workspace, at most the whole row length rounded up
+template <class T>
+int64_t core_zcols(const SigmaPlan& P, const CoreTable& ct, int64_t ncols) {
+  const int64_t whole = (ncols + CORE_COLS - 1) / CORE_COLS * CORE_COLS;
+  if (ct.nmem == 0) return whole;
+  const int64_t fit = (int64_t)(P.zbytes / (sizeof(T) * (size_t)ct.nmem)) / CORE_COLS * CORE_COLS;
+  if (fit < CORE_COLS) throw Error(SBDG_ERR_LIMIT, "core workspace too small");
+  return std::min(fit, whole);
+}
+
+// products of one spin (rows of X are that spin's strings): singles + diagonal by the
+// excitation table, doubles by core, one pass per column range of Z
+template <class T>
+void launch_spin(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, const CoreTable* ct, const T* W,
+                 int64_t npair, const T* X, const T* hii, T* out, T* Z, const SigmaPlan& P, hipStream_t st) {
+  if (!ct) {
+    launch_spmm<T>(diag, nrows, ncols, tb, X, hii, out, st);
+    return;
+  }
+  if (nrows == 0 || ncols == 0) return;
+  constexpr int VN = V16<T>::n;
+  const int64_t zcols = core_zcols<T>(P, *ct, ncols);
+  const bool vec = (ncols % VN) == 0;
+  for (int64_t cb = 0; cb < ncols; cb += zcols) {
+    const int64_t ce = std::min(ncols, cb + zcols);
+    if (ct->ngrp > 0) {
+      const int64_t ntiles = (ce - cb + TS * VN - 1) / (TS * VN);
+      if (ntiles * ct->ngrp > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
+      dim3 g((unsigned)(ntiles * ct->ngrp)), b(TS);
+      if (vec)
+        hipLaunchKernelGGL((k_core_prod<T, true>), g, b, 0, st, ncols, cb, ce, ct->start.p, ct->row.p, ct->pair.p,
+                           ct->blk.p, ct->blk_x0.p, ct->grp.p, W, npair, X, Z, zcols);
+      else
+        hipLaunchKernelGGL((k_core_prod<T, false>), g, b, 0, st, ncols, cb, ce, ct->start.p, ct->row.p,
+                           ct->pair.p, ct->blk.p, ct->blk_x0.p, ct->grp.p, W, npair, X, Z, zcols);
+    }
+    launch_spmm<T>(diag, nrows, ncols, tb, X, hii, out, st, cb, ce, ct, Z, zcols);
   }
 }
 
@@ -351,9 +536,13 @@ void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream
   if (ev) SBDG_CHECK(hipEventRecord(ev[0], st));
   launch_transpose<T, false>(P.na, P.nb, c, cT, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[1], st));
-  launch_spmm<T>(false, P.nb, P.na, *P.mb, cT, nullptr, sT, st);
+  const T* W;
+  if constexpr (sizeof(T) == 8) W = reinterpret_cast<const T*>(P.W);
+  else W = reinterpret_cast<const T*>(P.W32);
+  T* Z = reinterpret_cast<T*>(P.Z);
+  launch_spin<T>(false, P.nb, P.na, *P.mb, P.cb, W, P.npair, cT, nullptr, sT, Z, P, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[2], st));
-  launch_spmm<T>(true, P.na, P.nb, *P.ma, c, hii, sigma, st);
+  launch_spin<T>(true, P.na, P.nb, *P.ma, P.ca, W, P.npair, c, hii, sigma, Z, P, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[3], st));
   launch_transpose<T, true>(P.nb, P.na, sT, sigma, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[4], st));

@@ -6,6 +6,8 @@
 // ballot/prefix so the ascending order survives), then a per-entry pass derives the
 // orbitals with ctz and the separated-order sign with a masked popcount, and evaluates
 // the spin-string-only part of the Slater-Condon element (determinants.h:423-465).
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 
 #include "sbdg_internal.h"
@@ -269,7 +271,7 @@ __global__ void k_ell_count(int64_t npos, int nch, int64_t chunk, const int32_t*
   cnt[t] = n;
 }
 
-// K[g*nch+k] = max over the 64 lane positions of group g
+// K[g*nch+k] = max over the 64 lane positions of group g, rounded up to a multiple of 4
 __global__ void k_ell_max(int64_t groups, int nch, const int32_t* __restrict__ cnt,
                           int64_t* __restrict__ K64, int32_t* __restrict__ K) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,6 +280,7 @@ __global__ void k_ell_max(int64_t groups, int nch, const int32_t* __restrict__ c
   const int k = (int)(t % nch);
   int m = 0;
   for (int l = 0; l < 64; ++l) m = max(m, cnt[(g * 64 + l) * nch + k]);
+  m = (m + 3) & ~3;  // the kernel reads entries four at a time
   K[t] = m;
   K64[t] = (int64_t)m * 64;
 }
@@ -319,6 +322,53 @@ __global__ void k_ell_fill(int64_t groups, int nch, int64_t chunk, int L, const 
     }
   }
   for (; e < kk; ++e) ent[base + (int64_t)e * 64] = pad;
+}
+
+// memberships: thread t -> (string r, electron pair q); key = core bits, value =
+// r | P(a,b) << 32 | neg << 63 with eps = (-1)^(#core below a + #core below b)
+__global__ void k_core_enum(const uint64_t* __restrict__ s, int64_t n, int nel, uint64_t* __restrict__ key,
+                            uint64_t* __restrict__ val) {
+  const int64_t npp = (int64_t)nel * (nel - 1) / 2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * npp) return;
+  const int64_t r = t / npp;
+  int q = (int)(t % npp);
+  int j = 1;
+  while ((j + 1) * j / 2 <= q) ++j;  // positions i < j of the occupied orbitals
+  const int i = q - j * (j - 1) / 2;
+  uint64_t x = s[r];
+  int a = -1, b = -1;
+  for (int pos = 0; x; ++pos) {
+    const int o = __builtin_ctzll(x);
+    x &= x - 1;
+    if (pos == i) a = o;
+    if (pos == j) b = o;
+  }
+  const uint64_t core = s[r] & ~(1ull << a) & ~(1ull << b);
+  const int neg = (__popcll(core & ((1ull << a) - 1ull)) + __popcll(core & ((1ull << b) - 1ull))) & 1;
+  key[t] = core;
+  val[t] = (uint64_t)r | ((uint64_t)(b * (b - 1) / 2 + a) << 32) | ((uint64_t)neg << 63);
+}
+
+// W[P*npair+Q] = (AI|BJ) - (AJ|BI) for P=(A<B), Q=(I<J) disjoint, 0 otherwise
+__global__ void k_pair_int(const double* __restrict__ eri, int L, int64_t npair, double* __restrict__ W,
+                           float* __restrict__ W32) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npair * npair) return;
+  const int64_t P = t / npair, Q = t % npair;
+  auto unpair = [](int64_t p, int& lo, int& hi) {
+    int b = 1;
+    while ((int64_t)(b + 1) * b / 2 <= p) ++b;
+    hi = b;
+    lo = (int)(p - (int64_t)b * (b - 1) / 2);
+  };
+  int A, B, I, J;
+  unpair(P, A, B);
+  unpair(Q, I, J);
+  double v = 0.0;
+  if (A != I && A != J && B != I && B != J) v = eri4(eri, L, A, I, B, J) - eri4(eri, L, A, J, B, I);
+  W[t] = v;
+  W32[t] = (float)v;
 }
 
 // per-string diagonal parts: E[s] = sum_{i in s} h_ii + sum_{i<j in s} [(ii|jj)-(ij|ji)]
@@ -418,7 +468,23 @@ void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, co
   SBDG_CHECK(hipStreamSynchronize(st));
 }
 
-void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st) {
+void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st, bool with_doubles) {
+  if (!with_doubles) {  // singles only: copy the singles CSR
+    m.n = s.n;
+    m.nnz = s.nnz;
+    m.rowptr.alloc(m.n + 1);
+    m.col.alloc(std::max<int64_t>(m.nnz, 1));
+    m.val.alloc(std::max<int64_t>(m.nnz, 1));
+    m.val32.alloc(std::max<int64_t>(m.nnz, 1));
+    SBDG_CHECK(hipMemcpyAsync(m.rowptr.p, s.rowptr.p, sizeof(int64_t) * (m.n + 1), hipMemcpyDeviceToDevice, st));
+    if (m.nnz) {
+      SBDG_CHECK(hipMemcpyAsync(m.col.p, s.col.p, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToDevice, st));
+      SBDG_CHECK(hipMemcpyAsync(m.val.p, s.val.p, sizeof(double) * m.nnz, hipMemcpyDeviceToDevice, st));
+      SBDG_CHECK(hipMemcpyAsync(m.val32.p, s.val32.p, sizeof(float) * m.nnz, hipMemcpyDeviceToDevice, st));
+    }
+    SBDG_CHECK(hipStreamSynchronize(st));
+    return;
+  }
   m.n = s.n;
   m.nnz = s.nnz + d.nnz;
   m.rowptr.alloc(m.n + 1);
@@ -488,6 +554,95 @@ void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, EllTa
   if (ng > 0)
     hipLaunchKernelGGL(k_ell_fill, dim3(blocks_for(ng * 64, 256)), dim3(256), 0, st, e.groups, e.nch, chunk, L,
                        e.colperm.p, sb.rowptr.p, sb.col.p, sb.exc.p, d_b, e.off.p, e.K.p, e.ent.p);
+  SBDG_CHECK(hipGetLastError());
+  SBDG_CHECK(hipStreamSynchronize(st));
+}
+
+void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct, hipStream_t st) {
+  ct.ncore = ct.nmem = ct.nblk = 0;
+  ct.maxm = 0;
+  const int64_t npp = (int64_t)nel * (nel - 1) / 2;
+  const int64_t total = n * npp;
+  std::vector<int32_t> start(1, 0), row, pair, blk, blk_x0, grp, slot_ptr(n + 1, 0), slot;
+  if (total > 0) {
+    if (total >= (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "too many core memberships");
+    DBuf<uint64_t> key, val, key2, val2;
+    key.alloc(total);
+    val.alloc(total);
+    key2.alloc(total);
+    val2.alloc(total);
+    hipLaunchKernelGGL(k_core_enum, dim3(blocks_for(total, 256)), dim3(256), 0, st, d_str, n, nel, key.p, val.p);
+    SBDG_CHECK(hipGetLastError());
+    size_t tmp = 0;
+    SBDG_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key.p, key2.p, val.p, val2.p, (int)total, 0,
+                                                  std::max(1, L), st));
+    DBuf<unsigned char> tb;
+    tb.alloc(tmp + 1);
+    SBDG_CHECK(hipcub::DeviceRadixSort::SortPairs(tb.p, tmp, key.p, key2.p, val.p, val2.p, (int)total, 0,
+                                                  std::max(1, L), st));
+    std::vector<uint64_t> hk(total), hv(total);
+    SBDG_CHECK(hipMemcpyAsync(hk.data(), key2.p, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, st));
+    SBDG_CHECK(hipMemcpyAsync(hv.data(), val2.p, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, st));
+    SBDG_CHECK(hipStreamSynchronize(st));
+    // keep cores with >= 2 members (radix sort is stable: members ascend by string)
+    for (int64_t e = 0; e < total;) {
+      int64_t f = e + 1;
+      while (f < total && hk[f] == hk[e]) ++f;
+      const int64_t m = f - e;
+      if (m >= 2) {
+        const int32_t c = (int32_t)(start.size() - 1);
+        for (int64_t x0 = 0; x0 < m; x0 += CORE_XB) {
+          blk.push_back(c);
+          blk_x0.push_back((int32_t)x0);
+        }
+        for (int64_t q = e; q < f; ++q) {
+          row.push_back((int32_t)(hv[q] & 0xFFFFFFFFull));
+          pair.push_back((int32_t)((hv[q] >> 32) & 0x7FFFFFFFull) | (int32_t)((hv[q] >> 63) << 31));
+        }
+        start.push_back((int32_t)row.size());
+        ct.maxm = std::max<int>(ct.maxm, (int)m);
+      }
+      e = f;
+    }
+    // workgroups: consecutive product blocks until about CORE_WORK member rows
+    grp.push_back(0);
+    int64_t work = 0;
+    for (size_t q = 0; q < blk.size(); ++q) {
+      work += start[blk[q] + 1] - start[blk[q]];
+      if (work >= CORE_WORK || q + 1 == blk.size()) {
+        grp.push_back((int32_t)(q + 1));
+        work = 0;
+      }
+    }
+    // per-string slots (counting sort by row)
+    for (int32_t r : row) ++slot_ptr[r + 1];
+    for (int64_t r = 0; r < n; ++r) slot_ptr[r + 1] += slot_ptr[r];
+    slot.resize(row.size());
+    std::vector<int32_t> fillp(slot_ptr.begin(), slot_ptr.end() - 1);
+    for (size_t q = 0; q < row.size(); ++q) slot[fillp[row[q]]++] = (int32_t)q;
+  }
+  ct.ncore = (int64_t)start.size() - 1;
+  ct.nmem = (int64_t)row.size();
+  ct.nblk = (int64_t)blk.size();
+  ct.ngrp = grp.empty() ? 0 : (int64_t)grp.size() - 1;
+  auto up = [&](DBuf<int32_t>& d, const std::vector<int32_t>& h) {
+    d.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) SBDG_CHECK(hipMemcpy(d.p, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+  };
+  up(ct.start, start);
+  up(ct.row, row);
+  up(ct.pair, pair);
+  up(ct.blk, blk);
+  up(ct.blk_x0, blk_x0);
+  up(ct.slot_ptr, slot_ptr);
+  up(ct.slot, slot);
+  up(ct.grp, grp);
+}
+
+void build_pair_integrals(const double* d_eri, int L, double* W, float* W32, hipStream_t st) {
+  const int64_t npair = (int64_t)L * (L - 1) / 2;
+  if (npair == 0) return;
+  hipLaunchKernelGGL(k_pair_int, dim3(blocks_for(npair * npair, 256)), dim3(256), 0, st, d_eri, L, npair, W, W32);
   SBDG_CHECK(hipGetLastError());
   SBDG_CHECK(hipStreamSynchronize(st));
 }

@@ -109,10 +109,12 @@ def test_pair_orbitals_and_signs():
 
 # ---------------------------------------------------------------- sigma
 
+@pytest.mark.parametrize("doubles", [1, 2])
 @pytest.mark.parametrize("name", SMALL_CASES)
-def test_hii_and_sigma_vs_reference(name):
+def test_hii_and_sigma_vs_reference(name, doubles):
+    """doubles=1: same-spin doubles from the excitation tables; 2: grouped by core."""
     g, case = small_case(name)
-    s = solver(case)
+    s = solver(case, doubles=doubles)
     assert np.abs(s.hii() - g["hii"]).max() < 1e-13
     vec = np.random.default_rng(int(g["vec_seed"])).normal(size=s.ndet)
     sig = s.sigma(vec)
@@ -121,10 +123,12 @@ def test_hii_and_sigma_vs_reference(name):
     assert np.abs(sig32 - g["sigma"]).max() <= 2e-5 * np.abs(g["sigma"]).max()
 
 
-def test_sigma_columns_are_matrix_columns():
-    """sigma(e_J) is column J of H: every element (sign included) of those columns."""
-    case = workload_small(8, 3, 2, 30, 20, 51)
-    s = solver(case)
+@pytest.mark.parametrize("doubles", [1, 2])
+def test_sigma_columns_are_matrix_columns(doubles):
+    """sigma(e_J) is column J of H: every element (sign included) of those columns.
+    21 beta strings: row lengths that are not a multiple of the vector width."""
+    case = workload_small(8, 3, 2, 30, 21, 51)
+    s = solver(case, doubles=doubles)
     H = bf.hamiltonian(case["norb"], case["core"], case["h1"], case["eri"], case["astr"], case["bstr"])
     for J in (0, 1, 37, 199, s.ndet - 1):
         e = np.zeros(s.ndet)
@@ -132,11 +136,14 @@ def test_sigma_columns_are_matrix_columns():
         col = s.sigma(e)
         assert np.abs(col - H[:, J]).max() < 1e-13
         assert np.array_equal(np.sign(np.round(col, 12)), np.sign(np.round(H[:, J], 12)))
+        col32 = s.sigma(e, precision=1)
+        assert np.abs(col32 - H[:, J]).max() < 2e-5 * np.abs(H[:, J]).max()
 
 
-def test_sigma_medium_vs_oracle_and_symmetry():
+@pytest.mark.parametrize("doubles", [1, 2])
+def test_sigma_medium_vs_oracle_and_symmetry(doubles):
     case = medium_case()
-    s, p = solver(case), oproblem(case)
+    s, p = solver(case, doubles=doubles), oproblem(case)
     rng = np.random.default_rng(2)
     x, y = rng.normal(size=s.ndet), rng.normal(size=s.ndet)
     sx, ox = s.sigma(x), p.sigma(x)
@@ -147,12 +154,14 @@ def test_sigma_medium_vs_oracle_and_symmetry():
     assert np.abs(sxy - (2.0 * sx - 3.0 * sy)).max() < 1e-11 * np.abs(sxy).max()  # linear
 
 
-def test_sigma_c1_shape_row_sample():
+@pytest.mark.parametrize("doubles,core_gib", [(1, 8.0), (2, 8.0), (2, 0.2)])
+def test_sigma_c1_shape_row_sample(doubles, core_gib):
     """Full c1 shape (norb=26, 5+5, 4000 x 4000 strings): GPU tables equal the oracle's,
-    and sigma on a sample of rows equals the oracle restricted to those rows."""
+    and sigma on a sample of rows equals the oracle restricted to those rows.  A 0.2 GiB
+    core workspace splits the core products into several 512-column passes."""
     wl = syn.config_workload("c1_L26_5e5e_4000")
     case = dict(norb=wl.norb, core=wl.core, h1=wl.h1, eri=wl.eri, astr=wl.astr, bstr=wl.bstr)
-    s, p = solver(case), oproblem(case)
+    s, p = solver(case, doubles=doubles, core_gib=core_gib), oproblem(case)
     for which in range(4):
         rp, col, _, _ = s.table(which)
         rpo, colo = p.table(which)
@@ -164,6 +173,19 @@ def test_sigma_c1_shape_row_sample():
         o = p.sigma_rows(x, hii, r0, r0 + 2)
         blk = slice(r0 * s.nb, (r0 + 2) * s.nb)
         assert np.abs(sx[blk] - o[blk]).max() <= 1e-12 * np.abs(o[blk]).max()
+    # FP32 sigma on the same rows
+    sx32 = s.sigma(x, precision=1)
+    for r0 in (0, 3998):
+        blk = slice(r0 * s.nb, (r0 + 2) * s.nb)
+        assert np.abs(sx32[blk] - sx[blk]).max() <= 2e-5 * np.abs(sx[blk]).max()
+
+
+def test_core_doubles_energy_equals_table_doubles():
+    """Davidson energies with either doubles formulation agree to round-off."""
+    case = medium_case()
+    e1 = solver(case, doubles=1, max_it=50, max_nb=10, eps=1e-9).run()
+    e2 = solver(case, doubles=2, max_it=50, max_nb=10, eps=1e-9).run()
+    assert abs(e1 - e2) < 1e-10
 
 
 # ---------------------------------------------------------------- Davidson, RDMs
