@@ -27,26 +27,58 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sbd-ykt_amd"))
 
 HBM_PEAK_GBS = 8000.0
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
 
 
-def kernel_bytes(na, nb, nnz, s):
-    """Algorithmic HBM bytes per launch of each sigma kernel (s = bytes per element).
+def measured_traffic(kernel):
+    """HBM bytes per launch of a sigma phase from the committed counter summary
+    (FETCH_SIZE scaled by the copy-kernel calibration, plus WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", "sigma_c2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    v = t.get("per_launch_bytes", {}).get(kernel)
+    return None if v is None else int(v)
 
-    transpose:    read c, write c^T                         2 N s
-    beta product: stream c^T rows: nnz_b rows of na, write  s na (nnzSb+nnzDb) + N s + 12 nnz_b
-    alpha product: hii, c, nnz_a rows of nb, write          s nb (nnzSa+nnzDa) + 3 N s + 12 nnz_a
-    transpose-add: read sigma^T and sigma, write sigma      3 N s
-    alpha-beta:   one ket row per (ia, ja) pair incl. ia    s nb (na + nnzSa) + 2 N s + 4 nnzSb
+
+def kernel_bytes(na, nb, nnz, core, nel_b, s):
+    """Algorithmic HBM bytes per launch of each sigma phase (s = bytes per element).
+
+    transpose:     read c, write c^T                                   2 N s
+    beta product:  on c^T: singles rows + diagonal-free write          s na nnzSb + N s + 12 nnzSb
+                   doubles by core (mb memberships): member rows in,
+                   partial rows out, partial rows gathered              + 3 s na mb
+                   (excitation-table form instead: s na nnzDb + 12 nnzDb)
+    alpha product: same on c, with hii and c read for the diagonal     s nb nnzSa + 3 N s + ...
+    transpose-add: read sigma^T and sigma, write sigma                 3 N s
+    alpha-beta:    one ket row per (ia, ja) pair incl. ia, sigma
+                   read and written, ELL entries                       s nb (na + nnzSa) + 2 N s + 4 (nnzSb + nb nel_b)
     """
     N = na * nb
     sa, da, sb, db = nnz
+    ma, mb = core
+
+    def spin(ncols, singles, doubles, members):
+        if members:
+            return s * ncols * (singles + 3 * members) + 12 * singles
+        return s * ncols * (singles + doubles) + 12 * (singles + doubles)
+
     return {
         "transpose": 2 * N * s,
-        "beta": s * na * (sb + db) + N * s + 12 * (sb + db),
-        "alpha": s * nb * (sa + da) + 3 * N * s + 12 * (sa + da),
+        "beta": spin(na, sb, db, mb) + N * s,
+        "alpha": spin(nb, sa, da, ma) + 3 * N * s,
         "transpose_add": 3 * N * s,
-        "alpha_beta": s * nb * (na + sa) + 2 * N * s + 4 * sb,
+        "alpha_beta": s * nb * (na + sa) + 2 * N * s + 4 * (sb + nb * nel_b),
     }
+
+
+def ab_lds_bytes(na, nb, nnz, nel_b, s):
+    """LDS bytes the alpha-beta kernel reads per launch: every (bra row, ket row) pair of
+    the alpha singles plus the row itself, times every ELL entry (beta singles plus one
+    occupation entry per beta electron), one ket value and one integral each."""
+    sa, _, sb, _ = nnz
+    return (sa + na) * (sb + nb * nel_b) * 2 * s
 
 
 def survey_b_alg(na, nb, nnz, s):
@@ -139,13 +171,19 @@ def main():
     print(f"[bench] fp32 sigma {ms32[0]:.2f} ms/step", flush=True)
 
     names = ["transpose", "beta", "alpha", "transpose_add", "alpha_beta"]
-    kb = kernel_bytes(na, nb, nnz, 8)
+    nel_b = bin(int(wl.bstr[0])).count("1")
+    core = st["core_members"]
+    kb = kernel_bytes(na, nb, nnz, core, nel_b, 8)
     per = {n: float(ms[i + 1]) for i, n in enumerate(names)}
     dom = max(per, key=per.get)
     achieved = kb[dom] / (per[dom] * 1e-3) / 1e9
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom),
                     launch_ms=round(per[dom], 3), alg_bytes=int(kb[dom]))
+    if dom == "alpha_beta":
+        lb = ab_lds_bytes(na, nb, nnz, nel_b, 8)
+        roofline["lds"] = dict(bytes=int(lb), achieved_TBs=round(lb / (per[dom] * 1e-3) / 1e12, 2),
+                               peak_TBs=LDS_PEAK_TBS, frac=round(lb / (per[dom] * 1e-3) / 1e12 / LDS_PEAK_TBS, 4))
     b_alg = survey_b_alg(na, nb, nnz, 8)
     sigma_model = dict(alg_bytes=int(b_alg), achieved_GBs=round(b_alg / (ms[0] * 1e-3) / 1e9, 1),
                        frac=round(b_alg / (ms[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -74,6 +74,8 @@ typedef struct {
   double t_rdm;         /* seconds: RDMs or density */
   int64_t terms;        /* structurally connected (bra, ket) pairs per sigma, diagonal included */
   int64_t nnz[4];       /* alpha singles, alpha doubles, beta singles, beta doubles */
+  int64_t core_members[2]; /* alpha, beta: memberships of the core-grouped doubles (0: the
+                              excitation-table form is used for that spin) */
 } sbdg_stats;
 
 /* Context on a device.  Replaces the communicator set-up of TaskCommunicator

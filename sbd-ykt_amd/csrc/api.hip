@@ -180,8 +180,8 @@ void build(sbdg_ctx* c) {
   }
   build_ab(c->Sb, L, std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - (int64_t)L * L)),
            c->ab64, c->st);
-  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 8), c->ell64, c->st);
-  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 4), c->ell32, c->st);
+  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 8), 8, c->ell64, c->st);
+  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 4), 4, c->ell32, c->st);
   c->d_hii.alloc(std::max<int64_t>(N, 1));
   c->d_hii32.alloc(std::max<int64_t>(N, 1));
   compute_hii(c->d_a.p, na, c->d_b.p, nb, L, c->core, c->d_h1.p, c->d_eri.p, c->d_hii.p, c->st);
@@ -195,6 +195,8 @@ void build(sbdg_ctx* c) {
   c->stats.nnz[1] = c->Da.nnz;
   c->stats.nnz[2] = c->Sb.nnz;
   c->stats.nnz[3] = c->Db.nnz;
+  c->stats.core_members[0] = c->use_ca ? c->Ca.nmem : 0;
+  c->stats.core_members[1] = c->use_cb ? c->Cb.nmem : 0;
   c->stats.terms = N + (c->Sa.nnz + c->Da.nnz) * nb + (c->Sb.nnz + c->Db.nnz) * na + c->Sa.nnz * c->Sb.nnz;
 }
 
@@ -321,6 +323,7 @@ void davidson(sbdg_ctx* c) {
     c->stats.t_tables = keep_st.t_tables;
     c->stats.terms = keep_st.terms;
     for (int k = 0; k < 4; ++k) c->stats.nnz[k] = keep_st.nnz[k];
+    for (int k = 0; k < 2; ++k) c->stats.core_members[k] = keep_st.core_members[k];
   }
   initial_vector(c, c->V.p);
   h2d(one_dev, 1.0);

@@ -104,11 +104,12 @@ struct AbTable {
 // and the padding stays small when the connectivity is uneven (sampled string sets).
 struct EllTable {
   int nch = 0;
-  int64_t chunk = 0, groups = 0;
-  DBuf<int64_t> off;  // groups*nch + 1
-  DBuf<int32_t> K;    // groups*nch
+  int nt = 0, m = 0;        // alpha-beta launch: threads per workgroup, columns per thread
+  int64_t chunk = 0, groups = 0, ntiles = 0;
+  DBuf<int64_t> off;        // groups*nch + 1
+  DBuf<int32_t> K;          // groups*nch
   DBuf<uint32_t> ent;
-  DBuf<int32_t> colperm;  // groups*64 (positions beyond nb: -1)
+  DBuf<int32_t> colperm;    // nch * groups*64: column of each lane position in each chunk (-1: none)
 };
 
 // Double excitations of one spin grouped by their (n-2)-electron core: a double pair
@@ -142,7 +143,7 @@ void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct
 // W[P*npair + Q] (FP64 and FP32), npair = L(L-1)/2
 void build_pair_integrals(const double* d_eri, int L, double* W, float* W32, hipStream_t st);
 void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st);
-void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, EllTable& e, hipStream_t st);
+void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st);
 void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
                  const double* d_h1, const double* d_eri, double* d_hii, hipStream_t st);
 

@@ -292,6 +292,54 @@ __device__ inline void ab_slice(bool self, uint64_t A, int ai, int L, int64_t LL
   }
 }
 
+// Entries of one lane in one (group, chunk): sum_e sign_e * V[pair_e] * cb[jb_e] for the two
+// staged rows at once.  n is wave-uniform; four entries are in flight while the previous
+// four are used.
+template <class T>
+__device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const typename Pair2<T>::t* cbp,
+                                  const typename Pair2<T>::t* Vp, T& s0, T& s1) {
+  using T2 = typename Pair2<T>::t;
+  int e = 0;
+  if (n >= 4) {
+    uint32_t pn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pn[u] = ep[(uint32_t)u * 64u];
+    for (; e + 4 <= n; e += 4) {
+      uint32_t p[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) p[u] = pn[u];
+      if (e + 8 <= n) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pn[u] = ep[(uint32_t)(e + 4 + u) * 64u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const T2 v = Vp[(p[u] >> 18) & 0xFFFu];
+        const T2 x = cbp[p[u] & 0x3FFFFu];
+        const uint32_t neg = (p[u] >> 30) & 1u;
+        s0 += Pair2<T>::flip(v.x, neg) * x.x;
+        s1 += Pair2<T>::flip(v.y, neg) * x.y;
+      }
+    }
+  }
+  for (; e < n; ++e) {
+    const uint32_t p = ep[(uint32_t)e * 64u];
+    const T2 v = Vp[(p >> 18) & 0xFFFu];
+    const T2 x = cbp[p & 0x3FFFFu];
+    const uint32_t neg = (p >> 30) & 1u;
+    s0 += Pair2<T>::flip(v.x, neg) * x.x;
+    s1 += Pair2<T>::flip(v.y, neg) * x.y;
+  }
+}
+
+// Alpha-beta and cross-spin single terms of one bra row ia over one column tile:
+//   sigma[ia, ib] += sum_{ja in S(ia)+{ia}} coef * sum_{entries (jb, b<-j) of ib} sb V_ja[b*L+j] c[ja, jb]
+// with V_ja the (ai|..) slice of the alpha single ja (coef its sign) or, for ja = ia, the
+// alpha-occupation slice sum_{k in occ(ia)} (kk|..) (off-diagonal part).  Chunk by chunk of
+// ket columns: two ket rows of the chunk and their slices sit in LDS as interleaved pairs,
+// each lane walks its ELL entries for that chunk (the lane -> column map is per chunk, so
+// lanes of a group carry lists of similar length), and the chunk's sums are added into
+// sigma before the next chunk (one owner per element: a column belongs to one tile).
 template <class T, int M>
 __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, const uint64_t* __restrict__ astr,
                                              const int64_t* __restrict__ rpSa,
@@ -304,89 +352,87 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
                                              T* __restrict__ sigma) {
   using T2 = typename Pair2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  T2* cbp = reinterpret_cast<T2*>(smem);  // chunk + 1 pairs
+  T2* cbp = reinterpret_cast<T2*>(smem);  // chunk + 1 pairs (the last one stays zero: padding)
   T2* Vp = cbp + chunk + 1;               // L*L pairs
-  T* cbs = reinterpret_cast<T*>(cbp);
   T* Vs = reinterpret_cast<T*>(Vp);
   const int64_t LL = (int64_t)L * L;
   const int NT = blockDim.x;
+  const int64_t npos = groups * 64;
   const int64_t ia = blockIdx.x / ntiles;
   const int64_t col0 = (blockIdx.x - ia * ntiles) * (int64_t)NT * M;
   const int lane = threadIdx.x & 63;
+  const int64_t gw = (col0 >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   if (threadIdx.x == 0) cbp[chunk] = T2{T(0), T(0)};
-  T acc[M];
-#pragma unroll
-  for (int m = 0; m < M; ++m) acc[m] = T(0);
   const int64_t q0 = rpSa[ia], q1 = rpSa[ia + 1];
   const uint64_t A = astr[ia];
-  for (int64_t t = q0 - 1; t < q1; t += 2) {
-    const bool self = (t < q0);
-    const bool two = (t + 1 < q1);
-    int64_t ja0 = ia, ja1 = ia;
-    T coef0 = T(1), coef1 = T(0);
-    int ai0 = 0, ai1 = 0;
-    if (!self) {
-      ja0 = colSa[t];
-      const uint32_t e = excSa[t];
-      ai0 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);  // a*L + i
-      coef0 = ((e >> 24) & 1u) ? T(-1) : T(1);
-    }
-    if (two) {
-      ja1 = colSa[t + 1];
-      const uint32_t e = excSa[t + 1];
-      ai1 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);
-      coef1 = ((e >> 24) & 1u) ? T(-1) : T(1);
-    }
-    __syncthreads();
-    ab_slice<T>(self, A, ai0, L, LL, eri, Vs, 0);
-    if (two) ab_slice<T>(false, A, ai1, L, LL, eri, Vs, 1);
-    else
-      for (int x = threadIdx.x; x < LL; x += NT) Vs[2 * x + 1] = T(0);
-    const T* crow0 = c + ja0 * nb;
-    const T* crow1 = c + ja1 * nb;
-    for (int k = 0; k < nch; ++k) {
-      const int64_t j0 = (int64_t)k * chunk;
-      const int64_t nj = min(nb, j0 + chunk) - j0;
-      __syncthreads();
-      for (int64_t x = threadIdx.x; x < nj; x += NT) {
-        T2 v;
-        v.x = __builtin_nontemporal_load(crow0 + j0 + x);
-        v.y = two ? __builtin_nontemporal_load(crow1 + j0 + x) : T(0);
-        cbp[x] = v;
+  for (int k = 0; k < nch; ++k) {
+    const int64_t j0 = (int64_t)k * chunk;
+    const int64_t nj = min(nb, j0 + chunk) - j0;
+    T acc[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = T(0);
+    for (int64_t t = q0 - 1; t < q1; t += 2) {
+      const bool self = (t < q0);
+      const bool two = (t + 1 < q1);
+      int64_t ja0 = ia, ja1 = ia;
+      T coef0 = T(1), coef1 = T(0);
+      int ai0 = 0, ai1 = 0;
+      if (!self) {
+        ja0 = colSa[t];
+        const uint32_t e = excSa[t];
+        ai0 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);  // a*L + i
+        coef0 = ((e >> 24) & 1u) ? T(-1) : T(1);
+      }
+      if (two) {
+        ja1 = colSa[t + 1];
+        const uint32_t e = excSa[t + 1];
+        ai1 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);
+        coef1 = ((e >> 24) & 1u) ? T(-1) : T(1);
+      }
+      __syncthreads();  // previous pair done with LDS (and the previous chunk's sums are stored)
+      ab_slice<T>(self, A, ai0, L, LL, eri, Vs, 0);
+      if (two) ab_slice<T>(false, A, ai1, L, LL, eri, Vs, 1);
+      else
+        for (int x = threadIdx.x; x < LL; x += NT) Vs[2 * x + 1] = T(0);
+      const T* crow0 = c + ja0 * nb + j0;
+      const T* crow1 = c + ja1 * nb + j0;
+      for (int64_t x0 = threadIdx.x; x0 < nj; x0 += 4 * NT) {
+        T2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t x = x0 + (int64_t)u * NT;
+          v[u].x = x < nj ? __builtin_nontemporal_load(crow0 + x) : T(0);
+          v[u].y = (two && x < nj) ? __builtin_nontemporal_load(crow1 + x) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (x0 + (int64_t)u * NT < nj) cbp[x0 + (int64_t)u * NT] = v[u];
       }
       __syncthreads();
+      // the wave's group index, opaque to the compiler here: the per-group offsets and
+      // counts are then reloaded for every pair instead of being held in 3M registers
+      int gws = (int)gw;
+      asm volatile("" : "+s"(gws));
 #pragma unroll
       for (int m = 0; m < M; ++m) {
-        const int64_t g = (col0 + (int64_t)m * NT + threadIdx.x) >> 6;  // wave-uniform
+        const int64_t g = (int64_t)gws + (int64_t)m * (NT >> 6);  // group of this wave's m-th columns
         if (g < groups) {
           const int64_t idx = g * nch + k;
-          const int n = eK[idx];  // multiple of 4
-          const uint32_t* ep = ent + eoff[idx] + lane;
+          const int64_t o = eoff[idx];
+          const int n = eK[idx];
           T s0 = T(0), s1 = T(0);
-          for (int e = 0; e < n; e += 4) {
-            uint32_t p[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) p[u] = ep[(uint32_t)(e + u) * 64u];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const T2 v = Vp[(p[u] >> 18) & 0xFFFu];
-              const T2 x = cbp[p[u] & 0x3FFFFu];
-              const uint32_t neg = (p[u] >> 30) & 1u;
-              s0 += Pair2<T>::flip(v.x, neg) * x.x;
-              s1 += Pair2<T>::flip(v.y, neg) * x.y;
-            }
-          }
+          ab_entries<T>(ent + o + lane, n, cbp, Vp, s0, s1);
           acc[m] += coef0 * s0 + coef1 * s1;
         }
       }
     }
-  }
 #pragma unroll
-  for (int m = 0; m < M; ++m) {
-    const int64_t pos = col0 + (int64_t)m * NT + threadIdx.x;
-    if ((pos >> 6) < groups) {
-      const int32_t ib = colperm[pos];
-      if (ib >= 0) sigma[ia * nb + ib] += acc[m];
+    for (int m = 0; m < M; ++m) {
+      const int64_t pos = col0 + (int64_t)m * NT + threadIdx.x;
+      if ((pos >> 6) < groups) {
+        const int32_t ib = colperm[(int64_t)k * npos + pos];
+        if (ib >= 0) sigma[ia * nb + ib] += acc[m];
+      }
     }
   }
 }
@@ -480,12 +526,11 @@ void launch_transpose(int64_t R, int64_t C, const T* in, T* out, hipStream_t st)
 }
 
 template <class T, int M>
-void launch_ab_m(const SigmaPlan& P, int nt, size_t lds, const T* eri, const T* c, T* sigma, hipStream_t st) {
-  const int64_t ntiles = (P.nb + (int64_t)nt * M - 1) / ((int64_t)nt * M);
-  const int64_t nblk = ntiles * P.na;
-  if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
+void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* sigma, hipStream_t st) {
   const EllTable& E = *P.ell;
-  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(nt), lds, st, P.nb, P.L, ntiles, P.a,
+  const int64_t nblk = E.ntiles * P.na;
+  if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
+  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.nb, P.L, E.ntiles, P.a,
                      P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.off.p, E.K.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
                      E.groups, eri, c, sigma);
 }
@@ -493,19 +538,20 @@ void launch_ab_m(const SigmaPlan& P, int nt, size_t lds, const T* eri, const T* 
 template <class T>
 void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream_t st) {
   if (P.na == 0 || P.nb == 0) return;
-  const int nt = (int)std::min<int64_t>(1024, ((P.nb + 63) / 64) * 64);
-  const int64_t need = (P.nb + nt - 1) / nt;
-  const size_t lds = ab_lds_bytes(P.L, P.ell->chunk, (int)sizeof(T));
-  // one tile per row when the accumulators fit registers (the ket row is then staged once
-  // per pair); wider rows take several tiles of 24 columns per thread
-  if (need <= 1) launch_ab_m<T, 1>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 2) launch_ab_m<T, 2>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 4) launch_ab_m<T, 4>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 8) launch_ab_m<T, 8>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 12) launch_ab_m<T, 12>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 16) launch_ab_m<T, 16>(P, nt, lds, eri, c, sigma, st);
-  else if (need <= 20) launch_ab_m<T, 20>(P, nt, lds, eri, c, sigma, st);
-  else launch_ab_m<T, 24>(P, nt, lds, eri, c, sigma, st);
+  const EllTable& E = *P.ell;
+  if ((int64_t)E.nt * E.m * E.ntiles != E.groups * 64) throw Error(SBDG_ERR_STATE, "alpha-beta table geometry");
+  const size_t lds = ab_lds_bytes(P.L, E.chunk, (int)sizeof(T));
+  switch (E.m) {  // columns per thread, fixed when the table was built
+    case 1: launch_ab_m<T, 1>(P, lds, eri, c, sigma, st); break;
+    case 2: launch_ab_m<T, 2>(P, lds, eri, c, sigma, st); break;
+    case 4: launch_ab_m<T, 4>(P, lds, eri, c, sigma, st); break;
+    case 8: launch_ab_m<T, 8>(P, lds, eri, c, sigma, st); break;
+    case 12: launch_ab_m<T, 12>(P, lds, eri, c, sigma, st); break;
+    case 16: launch_ab_m<T, 16>(P, lds, eri, c, sigma, st); break;
+    case 20: launch_ab_m<T, 20>(P, lds, eri, c, sigma, st); break;
+    case 24: launch_ab_m<T, 24>(P, lds, eri, c, sigma, st); break;
+    default: throw Error(SBDG_ERR_STATE, "alpha-beta columns per thread");
+  }
 }
 
 }  // namespace

@@ -242,88 +242,9 @@ __global__ void k_abchunk(int64_t n, int nch, int64_t chunk, const int64_t* __re
   cptr[t] = (int32_t)lo;
 }
 
-__device__ inline int64_t lower_bound_col(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int64_t v) {
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if ((int64_t)col[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // cnt[ib*nch+k]: entries of beta row ib with ket in chunk k, plus the occupied orbitals of
 // ib when ib itself lies in chunk k (pseudo-entries)
-__global__ void k_ell_count(int64_t npos, int nch, int64_t chunk, const int32_t* __restrict__ perm,
-                            const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
-                            const uint64_t* __restrict__ bstr, int32_t* __restrict__ cnt) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npos * nch) return;
-  const int k = (int)(t % nch);
-  const int64_t ib = perm[t / nch];
-  if (ib < 0) {
-    cnt[t] = 0;
-    return;
-  }
-  const int64_t lo = lower_bound_col(col, rp[ib], rp[ib + 1], (int64_t)k * chunk);
-  const int64_t hi = lower_bound_col(col, lo, rp[ib + 1], (int64_t)(k + 1) * chunk);
-  int n = (int)(hi - lo);
-  if (ib / chunk == k) n += __popcll(bstr[ib]);
-  cnt[t] = n;
-}
-
-// K[g*nch+k] = max over the 64 lane positions of group g, rounded up to a multiple of 4
-__global__ void k_ell_max(int64_t groups, int nch, const int32_t* __restrict__ cnt,
-                          int64_t* __restrict__ K64, int32_t* __restrict__ K) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= groups * nch) return;
-  const int64_t g = t / nch;
-  const int k = (int)(t % nch);
-  int m = 0;
-  for (int l = 0; l < 64; ++l) m = max(m, cnt[(g * 64 + l) * nch + k]);
-  m = (m + 3) & ~3;  // the kernel reads entries four at a time
-  K[t] = m;
-  K64[t] = (int64_t)m * 64;
-}
-
-__global__ void k_ell_fill(int64_t groups, int nch, int64_t chunk, int L, const int32_t* __restrict__ perm,
-                           const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
-                           const uint32_t* __restrict__ exc, const uint64_t* __restrict__ bstr,
-                           const int64_t* __restrict__ off, const int32_t* __restrict__ K,
-                           uint32_t* __restrict__ ent) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= groups * 64 * nch) return;
-  const int64_t pos = t / nch;
-  const int k = (int)(t % nch);
-  const int64_t g = pos >> 6;
-  const int lane = (int)(pos & 63);
-  const int64_t ib = perm[pos];
-  const int64_t idx = g * nch + k;
-  const int64_t base = off[idx] + lane;
-  const int kk = K[idx];
-  const uint32_t pad = (uint32_t)chunk;
-  int e = 0;
-  if (ib >= 0) {
-    const int64_t j0 = (int64_t)k * chunk;
-    const int64_t lo = lower_bound_col(col, rp[ib], rp[ib + 1], j0);
-    const int64_t hi = lower_bound_col(col, lo, rp[ib + 1], j0 + chunk);
-    for (int64_t q = lo; q < hi; ++q, ++e) {
-      const uint32_t x = exc[q];
-      const uint32_t j = x & 63u, b = (x >> 6) & 63u, neg = (x >> 24) & 1u;
-      ent[base + (int64_t)e * 64] = (uint32_t)(col[q] - j0) | ((b * (uint32_t)L + j) << 18) | (neg << 30);
-    }
-    if (ib / chunk == k) {
-      uint64_t y = bstr[ib];
-      while (y) {
-        const uint32_t o = (uint32_t)__builtin_ctzll(y);
-        y &= y - 1;
-        ent[base + (int64_t)e * 64] = (uint32_t)(ib - j0) | ((o * (uint32_t)L + o) << 18);
-        ++e;
-      }
-    }
-  }
-  for (; e < kk; ++e) ent[base + (int64_t)e * 64] = pad;
-}
-
 // memberships: thread t -> (string r, electron pair q); key = core bits, value =
 // r | P(a,b) << 32 | neg << 63 with eps = (-1)^(#core below a + #core below b)
 __global__ void k_core_enum(const uint64_t* __restrict__ s, int64_t n, int nel, uint64_t* __restrict__ key,
@@ -518,44 +439,195 @@ void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st
   SBDG_CHECK(hipGetLastError());
 }
 
-void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, EllTable& e, hipStream_t st) {
+// Alpha-beta launch geometry: nt threads, m columns per thread, so that one workgroup
+// covers a whole row when m <= 24 (the ket row is then staged once per pair).
+static void ab_geometry(int64_t nb, int& nt, int& m) {
+  nt = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (nb + 63) / 64 * 64));
+  const int64_t need = (nb + nt - 1) / nt;
+  static const int ms[] = {1, 2, 4, 8, 12, 16, 20, 24};
+  m = 24;
+  for (int x : ms)
+    if (need <= x) {
+      m = x;
+      break;
+    }
+}
+
+// Order of the entries of the lanes of one (group, chunk) block, step by step, so that the
+// lanes of each aligned pass group (P lanes) read distinct LDS slots (S slots of the
+// element size) in both the ket-row array and the integral array; equal addresses
+// broadcast and cost nothing.  Padding (the zero slot, pair 0) may be placed at any step
+// while a lane has slack.  Greedy: at each step the lanes with least slack choose first,
+// each taking the entry that adds the fewest new addresses to slots already in use.
+// lists[l] holds lane l's entries (ket offset | pair << 18 | sign << 30); out gets K*64.
+static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int P, int S, uint32_t vbase,
+                           uint32_t pad, uint32_t* out) {
+  struct Use {
+    uint32_t addr[8];
+    int n;
+  };
+  std::vector<int> rem(64), order(P);
+  for (int l = 0; l < 64; ++l) rem[l] = (int)lists[l].size();
+  auto cost1 = [](const Use& u, uint32_t a) {
+    for (int i = 0; i < u.n; ++i)
+      if (u.addr[i] == a) return 0;
+    return u.n;
+  };
+  auto add1 = [](Use& u, uint32_t a) {
+    for (int i = 0; i < u.n; ++i)
+      if (u.addr[i] == a) return;
+    if (u.n < 8) u.addr[u.n++] = a;
+  };
+  std::vector<Use> uc(S), uv(S);
+  for (int p0 = 0; p0 < 64; p0 += P) {
+    for (int e = 0; e < K; ++e) {
+      for (int x = 0; x < S; ++x) uc[x].n = uv[x].n = 0;
+      for (int i = 0; i < P; ++i) order[i] = p0 + i;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rem[a] > rem[b]; });
+      for (int l : order) {
+        const int slack = (K - e) - rem[l];
+        auto& ls = lists[l];
+        int best = -1, bc = 1 << 30;
+        for (int j = 0; j < rem[l]; ++j) {
+          const uint32_t w = ls[j];
+          const uint32_t ca = w & 0x3FFFFu, va = vbase + ((w >> 18) & 0xFFFu);
+          const int cst = cost1(uc[ca % S], ca) + cost1(uv[va % S], va);
+          if (cst < bc) {
+            bc = cst;
+            best = j;
+            if (cst == 0) break;
+          }
+        }
+        uint32_t w = pad;
+        if (slack > 0) {
+          const uint32_t ca = pad & 0x3FFFFu, va = vbase + ((pad >> 18) & 0xFFFu);
+          const int cst = cost1(uc[ca % S], ca) + cost1(uv[va % S], va);
+          if (best < 0 || cst < bc) best = -1;
+        }
+        if (best >= 0) {
+          w = ls[best];
+          ls[best] = ls[rem[l] - 1];  // order of the rest does not matter
+          --rem[l];
+        }
+        const uint32_t ca = w & 0x3FFFFu, va = vbase + ((w >> 18) & 0xFFFu);
+        add1(uc[ca % S], ca);
+        add1(uv[va % S], va);
+        out[(size_t)e * 64 + l] = w;
+      }
+    }
+  }
+}
+
+void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st) {
   const int64_t nb = sb.n;
   if (chunk >= (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta chunk beyond 2^18 columns");
+  if (L > 64) throw Error(SBDG_ERR_LIMIT, "norb beyond 64");
   e.chunk = chunk;
   e.nch = (int)std::max<int64_t>(1, (nb + chunk - 1) / chunk);
-  e.groups = (nb + 63) / 64;
-  const int64_t npos = e.groups * 64, ng = e.groups * e.nch;
-  // lane positions: columns by decreasing number of beta singles (ties by index)
+  ab_geometry(nb, e.nt, e.m);
+  const int64_t tw = (int64_t)e.nt * e.m;
+  e.ntiles = std::max<int64_t>(1, (nb + tw - 1) / tw);
+  const int64_t npos = e.ntiles * tw;
+  e.groups = npos / 64;
+  const int nch = e.nch;
+  const int64_t ng = e.groups * nch;
+  // host copies of the beta singles
+  SBDG_CHECK(hipStreamSynchronize(st));
   std::vector<int64_t> rp(nb + 1);
   SBDG_CHECK(hipMemcpy(rp.data(), sb.rowptr.p, sizeof(int64_t) * (nb + 1), hipMemcpyDeviceToHost));
-  std::vector<int32_t> perm(std::max<int64_t>(npos, 1), -1);
-  for (int64_t i = 0; i < nb; ++i) perm[i] = (int32_t)i;
-  std::stable_sort(perm.begin(), perm.begin() + nb,
-                   [&](int32_t x, int32_t y) { return rp[x + 1] - rp[x] > rp[y + 1] - rp[y]; });
-  e.colperm.alloc(std::max<int64_t>(npos, 1));
-  SBDG_CHECK(hipMemcpy(e.colperm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
-  DBuf<int32_t> cnt;
-  DBuf<int64_t> K64;
-  cnt.alloc(std::max<int64_t>(npos * e.nch, 1));
-  K64.alloc(std::max<int64_t>(ng, 1));
-  e.K.alloc(std::max<int64_t>(ng, 1));
-  e.off.alloc(ng + 1);
-  if (nb > 0) {
-    hipLaunchKernelGGL(k_ell_count, dim3(blocks_for(npos * e.nch, 256)), dim3(256), 0, st, npos, e.nch, chunk,
-                       e.colperm.p, sb.rowptr.p, sb.col.p, d_b, cnt.p);
-    hipLaunchKernelGGL(k_ell_max, dim3(blocks_for(ng, 256)), dim3(256), 0, st, e.groups, e.nch, cnt.p, K64.p, e.K.p);
+  const int64_t nnz = rp[nb];
+  std::vector<int32_t> col(nnz);
+  std::vector<uint32_t> exc(nnz);
+  std::vector<uint64_t> bs(nb);
+  if (nnz) {
+    SBDG_CHECK(hipMemcpy(col.data(), sb.col.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    SBDG_CHECK(hipMemcpy(exc.data(), sb.exc.p, sizeof(uint32_t) * nnz, hipMemcpyDeviceToHost));
   }
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, K64.p, ng, e.off.p);
-  SBDG_CHECK(hipGetLastError());
-  int64_t total = 0;
-  SBDG_CHECK(hipMemcpyAsync(&total, e.off.p + ng, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  SBDG_CHECK(hipStreamSynchronize(st));
-  e.ent.alloc(std::max<int64_t>(total, 1));
-  if (ng > 0)
-    hipLaunchKernelGGL(k_ell_fill, dim3(blocks_for(ng * 64, 256)), dim3(256), 0, st, e.groups, e.nch, chunk, L,
-                       e.colperm.p, sb.rowptr.p, sb.col.p, sb.exc.p, d_b, e.off.p, e.K.p, e.ent.p);
-  SBDG_CHECK(hipGetLastError());
-  SBDG_CHECK(hipStreamSynchronize(st));
+  if (nb) SBDG_CHECK(hipMemcpy(bs.data(), d_b, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost));
+  // entries of column ib in chunk k: its singles with ket in the chunk, plus, in the chunk
+  // holding ib itself, one pseudo-entry per occupied orbital (alpha-occupation part of the
+  // cross-spin single, pair o*L+o)
+  std::vector<int32_t> cnt((size_t)nb * nch, 0);
+  for (int64_t ib = 0; ib < nb; ++ib) {
+    for (int64_t q = rp[ib]; q < rp[ib + 1]; ++q) ++cnt[(size_t)ib * nch + col[q] / chunk];
+    cnt[(size_t)ib * nch + ib / chunk] += __builtin_popcountll(bs[ib]);
+  }
+  // column tiles: columns by decreasing total count, cut into tiles of tw lane positions;
+  // inside a tile, each chunk orders its columns by decreasing count in that chunk, so the
+  // 64 lanes of a group carry similar lists (little padding).  A column stays in its tile
+  // for every chunk: one workgroup owns it.
+  std::vector<int32_t> order(nb);
+  for (int64_t i = 0; i < nb; ++i) order[i] = (int32_t)i;
+  std::vector<int32_t> tot(nb, 0);
+  for (int64_t ib = 0; ib < nb; ++ib)
+    for (int k = 0; k < nch; ++k) tot[ib] += cnt[(size_t)ib * nch + k];
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return tot[x] > tot[y]; });
+  std::vector<int32_t> perm((size_t)std::max<int64_t>(npos * nch, 1), -1);
+  for (int k = 0; k < nch; ++k) {
+    for (int64_t t = 0; t < e.ntiles; ++t) {
+      const int64_t lo = t * tw, hi = std::min<int64_t>(nb, lo + tw);
+      if (lo >= hi) break;
+      std::vector<int32_t> cols(order.begin() + lo, order.begin() + hi);
+      std::stable_sort(cols.begin(), cols.end(), [&](int32_t x, int32_t y) {
+        return cnt[(size_t)x * nch + k] > cnt[(size_t)y * nch + k];
+      });
+      for (size_t i = 0; i < cols.size(); ++i) perm[(size_t)k * npos + lo + i] = cols[i];
+    }
+  }
+  std::vector<int32_t> K((size_t)std::max<int64_t>(ng, 1), 0);
+  std::vector<int64_t> off((size_t)ng + 1, 0);
+  for (int64_t g = 0; g < e.groups; ++g)
+    for (int k = 0; k < nch; ++k) {
+      int mx = 0;
+      for (int l = 0; l < 64; ++l) {
+        const int32_t ib = perm[(size_t)k * npos + g * 64 + l];
+        if (ib >= 0) mx = std::max(mx, cnt[(size_t)ib * nch + k]);
+      }
+      K[g * nch + k] = mx;
+      off[g * nch + k + 1] = off[g * nch + k] + (int64_t)mx * 64;
+    }
+  const int64_t total = off[ng];
+  const uint32_t pad = (uint32_t)chunk;  // zero slot, pair 0
+  std::vector<uint32_t> ent((size_t)std::max<int64_t>(total, 1), pad);
+  // LDS pass groups of the kernel's pair reads: 16-byte pairs (FP64) in groups of 8 lanes
+  // over 8 slots, 8-byte pairs (FP32) in groups of 16 lanes over 16 slots
+  const int P = elem == 8 ? 8 : 16, S = P;
+  const uint32_t vbase = (uint32_t)(chunk + 1);  // integral pairs follow the ket pairs
+  std::vector<std::vector<uint32_t>> lists(64);
+  for (int64_t g = 0; g < e.groups; ++g)
+    for (int k = 0; k < nch; ++k) {
+      const int64_t j0 = (int64_t)k * chunk;
+      const int64_t base = off[g * nch + k];
+      for (int l = 0; l < 64; ++l) {
+        auto& lst = lists[l];
+        lst.clear();
+        const int32_t ib = perm[(size_t)k * npos + g * 64 + l];
+        if (ib < 0) continue;
+        for (int64_t q = rp[ib]; q < rp[ib + 1]; ++q) {
+          if (col[q] < j0 || col[q] >= j0 + chunk) continue;
+          const uint32_t x = exc[q];
+          const uint32_t j = x & 63u, b = (x >> 6) & 63u, neg = (x >> 24) & 1u;
+          lst.push_back((uint32_t)(col[q] - j0) | ((b * (uint32_t)L + j) << 18) | (neg << 30));
+        }
+        if (ib / chunk == k) {
+          uint64_t y = bs[ib];
+          while (y) {
+            const uint32_t o = (uint32_t)__builtin_ctzll(y);
+            y &= y - 1;
+            lst.push_back((uint32_t)(ib - j0) | ((o * (uint32_t)L + o) << 18));
+          }
+        }
+      }
+      schedule_block(lists, K[g * nch + k], P, S, vbase, pad, ent.data() + base);
+    }
+  e.colperm.alloc(perm.size());
+  SBDG_CHECK(hipMemcpy(e.colperm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
+  e.K.alloc(K.size());
+  SBDG_CHECK(hipMemcpy(e.K.p, K.data(), sizeof(int32_t) * K.size(), hipMemcpyHostToDevice));
+  e.off.alloc(off.size());
+  SBDG_CHECK(hipMemcpy(e.off.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
+  e.ent.alloc(ent.size());
+  SBDG_CHECK(hipMemcpy(e.ent.p, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
 }
 
 void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct, hipStream_t st) {
