@@ -13,8 +13,9 @@
 //     six digits, fcidump.h:156-169); SBD::round_integrals = 1 reproduces that rounding;
 //   * the mixed-spin 2-RDM blocks two_p_rdm[1] and [2] are the physical ones (the
 //     reference's sum two_p_rdm[1] + two_p_rdm[2] is unchanged);
-//   * wavefunction restart files (loadname / savename) and dump_matrix_form_wf are not
-//     supported: a non-empty name throws std::invalid_argument.
+//   * wavefunction files as one rank of the reference writes and reads them: savename
+//     writes <savename>000000, loadname reads <loadname>000000 (restart.h:20-67,
+//     414-657); dump_matrix_form_wf writes .bin / .txt / .dat (restart.h:662-780).
 // Errors from the library are rethrown as std::invalid_argument (bad input) or
 // std::runtime_error (everything else), as the reference throws.
 #ifndef SBD_TPB_MI355X_H
@@ -212,8 +213,6 @@ inline void diag(const SBD& o, const sbd::FCIDump& fcidump, const std::vector<st
                  const std::string& savename, double& energy, std::vector<double>& density,
                  std::vector<std::vector<size_t>>& carryover_bitstrings, std::vector<std::vector<double>>& one_p_rdm,
                  std::vector<std::vector<double>>& two_p_rdm) {
-  if (!loadname.empty() || !savename.empty() || !o.dump_matrix_form_wf.empty())
-    throw std::invalid_argument("wavefunction files are not supported by this build");
   if (o.method != 0) throw std::invalid_argument("only method 0 (direct sigma) is supported");
   int L = 0;
   for (const auto& kv : fcidump.header)
@@ -265,6 +264,8 @@ inline void diag(const SBD& o, const sbd::FCIDump& fcidump, const std::vector<st
   so.refresh_k = o.refresh_k;
   so.verbose = o.verbose;
   detail::check(c, sbdg_set_options(c, &so));
+  if (!loadname.empty())  // sbdiag.h:197-203
+    detail::check(c, sbdg_load_wavefunction(c, loadname.c_str(), (int)o.bit_length));
   detail::check(c, sbdg_run(c));
   detail::check(c, sbdg_get_energy(c, &energy, nullptr, nullptr));
   density.assign(2 * L, 0.0);
@@ -295,6 +296,10 @@ inline void diag(const SBD& o, const sbd::FCIDump& fcidump, const std::vector<st
       if ((co[k] >> i) & 1) w[i / o.bit_length] |= size_t(1) << (i % o.bit_length);
     carryover_bitstrings.push_back(w);
   }
+  if (!savename.empty())  // sbdiag.h:459-465
+    detail::check(c, sbdg_save_wavefunction(c, savename.c_str(), (int)o.bit_length));
+  if (!o.dump_matrix_form_wf.empty())  // sbdiag.h:468-474
+    detail::check(c, sbdg_save_matrix_form(c, o.dump_matrix_form_wf.c_str(), (int)o.bit_length));
 }
 
 // The reference signature with its communicator first (any type; ignored).

@@ -128,6 +128,26 @@ int sbdg_compute_rdm(sbdg_ctx *ctx);
  * Either output may be NULL. */
 int sbdg_rdm_backward(sbdg_ctx *ctx, double grad_e, double *dh1, double *deri);
 
+/* Wavefunction files of diag (restart.h).  bit_length is the reference's string word size
+ * (sbd::tpb::SBD::bit_length, 20 by default): orbital i sits in word i/bit_length at bit
+ * i%bit_length, ceil(norb/bit_length) words per string.
+ *
+ * sbdg_save_wavefunction replaces diag's savename (SaveWavefunction, restart.h:20-67) at
+ * one rank: file <prefix>000000 = size_t na, nb, words; the na alpha then nb beta strings
+ * as words; W (na*nb doubles, the vector of sbdg_get_vector).  Needs a solved context. */
+int sbdg_save_wavefunction(sbdg_ctx *ctx, const char *prefix, int bit_length);
+/* diag's loadname (LoadWavefunction, restart.h:414-657) at one rank: reads
+ * <prefix>000000, adds each coefficient whose alpha and beta strings are both in the
+ * current sets at their place, normalises, and makes it the initial vector (sets
+ * options.init = 2).  SBDG_ERR_NUMERIC when no coefficient lands in the basis (the
+ * reference would divide by a zero norm). */
+int sbdg_load_wavefunction(sbdg_ctx *ctx, const char *prefix, int bit_length);
+/* SaveMatrixFormWF (restart.h:662-780): extension .bin = na*nb doubles row-major; .txt or
+ * .dat = one line "<value> # <ia>: <alpha string> <ib>: <beta string>" per determinant,
+ * six significant digits, strings with orbital norb-1 first; other extensions write
+ * nothing. */
+int sbdg_save_matrix_form(sbdg_ctx *ctx, const char *filename, int bit_length);
+
 /* Carry-over alpha strings (CarryOverAlphaDet, tpb/rdmat.h:60-132): ratio > 0 keeps
  * floor(ratio*na) strings, else the smallest set of heaviest rows whose weight exceeds
  * 1 - threshold.  Writes up to cap strings; *count receives the number selected. */

@@ -6,9 +6,11 @@
 // the FCIDUMP, which would round every integral to six digits: fcidump.h:156-169).
 //
 //   sbd_ref diag  --fcidump F --adet A --bdet B --block nb --iteration it --tolerance eps
-//                 --rdm r --out PREFIX
-//       sbd::tpb::diag (sbdiag.h:113-477) -> PREFIX.energy.txt, PREFIX.wf.bin,
-//       PREFIX.rdm1.bin (2*L^2), PREFIX.rdm2.bin (4*L^4), PREFIX.density.bin (2*L)
+//                 --rdm r --out PREFIX [--load NAME] [--save NAME] [--dump_matrix_form_wf FILE]
+//       sbd::tpb::diag (sbdiag.h:113-477) -> PREFIX.energy.txt, PREFIX.wf.bin (unless
+//       another matrix-form file is named), PREFIX.rdm1.bin (2*L^2), PREFIX.rdm2.bin
+//       (4*L^4), PREFIX.density.bin (2*L); --load/--save are diag's loadname/savename
+//       (restart.h:20-67,414-657)
 //   sbd_ref sigma --fcidump F --adet A --bdet B --vec V --out PREFIX
 //       helpers (helper.h:447-589) + diagonal (tpb/qcham.h:314-366) + one direct mult
 //       (tpb/mult.h:376-632) -> PREFIX.hii.bin, PREFIX.sigma.bin, PREFIX.tables.bin
@@ -59,9 +61,10 @@ int main(int argc, char** argv) {
     std::vector<double> density;
     std::vector<std::vector<size_t>> cobits;
     std::vector<std::vector<double>> one, two;
-    sbd_data.dump_matrix_form_wf = out + ".wf.bin";
+    if (sbd_data.dump_matrix_form_wf.empty()) sbd_data.dump_matrix_form_wf = out + ".wf.bin";
+    const std::string loadname = arg(argc, argv, "--load", ""), savename = arg(argc, argv, "--save", "");
     auto t0 = std::chrono::high_resolution_clock::now();
-    sbd::tpb::diag(comm, sbd_data, fcidump, adet, bdet, "", "", energy, density, cobits, one, two);
+    sbd::tpb::diag(comm, sbd_data, fcidump, adet, bdet, loadname, savename, energy, density, cobits, one, two);
     auto t1 = std::chrono::high_resolution_clock::now();
     std::ofstream e(out + ".energy.txt");
     e.precision(17);

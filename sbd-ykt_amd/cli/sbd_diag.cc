@@ -1,7 +1,8 @@
 // sbd_diag -- command-line driver with the reference sample's interface
 // (samples/selected_basis_diagonalization/main.cc:17-226): the SBD options of
 // generate_sbd_data plus --fcidump, --adetfile (beta strings = alpha strings, as the
-// sample does), --bdetfile (separate beta strings), --carryoverfile, --rdmfiles 1.
+// sample does), --bdetfile (separate beta strings), --carryoverfile, --rdmfiles 1,
+// --loadname / --savename (wavefunction restart files).
 // Prints the energy, the density, the carry-over strings and, with --rdm 1, the
 // one-body + two-body energy identity.
 #include <cstdio>
@@ -22,6 +23,8 @@ int main(int argc, char** argv) {
     const std::string afile = arg(argc, argv, "--adetfile", "alphadets.txt");
     const std::string bfile = arg(argc, argv, "--bdetfile", "");
     const std::string cofile = arg(argc, argv, "--carryoverfile", "");
+    const std::string loadname = arg(argc, argv, "--loadname", "");  // main.cc:44-50
+    const std::string savename = arg(argc, argv, "--savename", "");
     auto fcidump = sbd::LoadFCIDump(fcifile);
     int L = 0;
     for (const auto& kv : fcidump.header)
@@ -34,7 +37,7 @@ int main(int argc, char** argv) {
     std::vector<double> density;
     std::vector<std::vector<size_t>> cobits;
     std::vector<std::vector<double>> one, two;
-    sbd::tpb::diag(o, fcidump, adet, bdet, "", "", energy, density, cobits, one, two);
+    sbd::tpb::diag(o, fcidump, adet, bdet, loadname, savename, energy, density, cobits, one, two);
     std::cout.precision(16);
     std::cout << " Sample-based diagonalization: Energy = " << energy << std::endl;
     std::cout << " Sample-based diagonalization: density = ";

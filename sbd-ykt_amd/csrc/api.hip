@@ -22,8 +22,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <random>
+#include <string>
+#include <unordered_map>
 
 #include "sbdg_internal.h"
 
@@ -542,6 +546,51 @@ void rdm_or_density(sbdg_ctx* c) {
 
 }  // namespace
 
+namespace {
+
+// A string as the reference's word vector: orbital i in word i / bit_length, bit
+// i % bit_length (from_string, bit_manipulation.h:158-172); words = ceil(norb/bit_length).
+std::vector<uint64_t> to_words(uint64_t s, int L, int bl) {
+  std::vector<uint64_t> w((size_t)((L + bl - 1) / bl), 0);
+  for (int i = 0; i < L; ++i)
+    if ((s >> i) & 1ull) w[(size_t)(i / bl)] |= 1ull << (i % bl);
+  return w;
+}
+
+// inverse of to_words for `words` words; false when a bit lies beyond orbital 63 or
+// beyond a word's bit_length (such a string cannot be in a set of this context)
+bool from_words(const uint64_t* w, size_t words, int bl, uint64_t& s) {
+  s = 0;
+  for (size_t k = 0; k < words; ++k) {
+    uint64_t x = w[k];
+    if (bl < 64 && (x >> bl)) return false;
+    while (x) {
+      const int p = __builtin_ctzll(x);
+      x &= x - 1;
+      const size_t orb = k * (size_t)bl + (size_t)p;
+      if (orb >= 64) return false;
+      s |= 1ull << orb;
+    }
+  }
+  return true;
+}
+
+std::vector<double> host_vector(sbdg_ctx* c) {  // W with interleaved-order phases
+  const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+  std::vector<double> w(N);
+  sign_transform(c->d_a.p, na, c->d_b.p, nb, c->W.p, c->T.p, c->st);
+  SBDG_CHECK(hipMemcpyAsync(w.data(), c->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+  SBDG_CHECK(hipStreamSynchronize(c->st));
+  return w;
+}
+
+int check_bit_length(sbdg_ctx* c, int bl) {
+  if (bl < 1 || bl > 64) return fail(c, SBDG_ERR_ARG, "bit_length must be in 1..64");
+  return SBDG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 sbdg_ctx* sbdg_create(int device) {
@@ -780,6 +829,126 @@ int sbdg_rdm_backward(sbdg_ctx* c, double g, double* dh1, double* deri) {
     if (dh1) SBDG_CHECK(hipMemcpyAsync(dh1, dh.p, sizeof(double) * L * L, hipMemcpyDeviceToHost, c->st));
     if (deri) SBDG_CHECK(hipMemcpyAsync(deri, de.p, sizeof(double) * L4, hipMemcpyDeviceToHost, c->st));
     SBDG_CHECK(hipStreamSynchronize(c->st));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_save_wavefunction(sbdg_ctx* c, const char* prefix, int bit_length) {
+  return guard(c, [&]() {
+    if (!c->solved) return fail(c, SBDG_ERR_STATE, "no solution");
+    if (!prefix) return fail(c, SBDG_ERR_ARG, "null file name");
+    if (check_bit_length(c, bit_length)) return SBDG_ERR_ARG;
+    const int L = c->L;
+    const std::vector<double> w = host_vector(c);
+    // SaveWavefunction (restart.h:20-67) at one rank: the file of rank 0
+    const std::string name = std::string(prefix) + "000000";
+    FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f) return fail(c, SBDG_ERR_ARG, "cannot open " + name);
+    const uint64_t na = c->a.size(), nb = c->b.size(), words = (uint64_t)((L + bit_length - 1) / bit_length);
+    bool ok = std::fwrite(&na, 8, 1, f) == 1 && std::fwrite(&nb, 8, 1, f) == 1 && std::fwrite(&words, 8, 1, f) == 1;
+    for (uint64_t x : c->a) ok = ok && std::fwrite(to_words(x, L, bit_length).data(), 8, words, f) == words;
+    for (uint64_t x : c->b) ok = ok && std::fwrite(to_words(x, L, bit_length).data(), 8, words, f) == words;
+    ok = ok && std::fwrite(w.data(), 8, w.size(), f) == w.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(c, SBDG_ERR_ARG, "write failed: " + name);
+    return SBDG_OK;
+  });
+}
+
+int sbdg_load_wavefunction(sbdg_ctx* c, const char* prefix, int bit_length) {
+  return guard(c, [&]() {
+    if (!c->have_strs) return fail(c, SBDG_ERR_STATE, "strings not set");
+    if (!prefix) return fail(c, SBDG_ERR_ARG, "null file name");
+    if (check_bit_length(c, bit_length)) return SBDG_ERR_ARG;
+    // LoadWavefunction (restart.h:414-657) at one rank: reads the file of rank 0
+    const std::string name = std::string(prefix) + "000000";
+    FILE* f = std::fopen(name.c_str(), "rb");
+    if (!f) return fail(c, SBDG_ERR_ARG, "failed to open input file " + name);
+    uint64_t hd[3] = {0, 0, 0};
+    bool ok = std::fread(hd, 8, 3, f) == 3;
+    const uint64_t la = hd[0], lb = hd[1], words = hd[2];
+    if (ok && (words == 0 || words > 64 || la > ((uint64_t)1 << 40) || lb > ((uint64_t)1 << 40))) ok = false;
+    std::vector<uint64_t> wa, wb;
+    std::vector<double> lw;
+    if (ok) {
+      wa.resize(la * words);
+      wb.resize(lb * words);
+      lw.resize(la * lb);
+      ok = std::fread(wa.data(), 8, wa.size(), f) == wa.size() && std::fread(wb.data(), 8, wb.size(), f) == wb.size() &&
+           std::fread(lw.data(), 8, lw.size(), f) == lw.size();
+    }
+    std::fclose(f);
+    if (!ok) return fail(c, SBDG_ERR_ARG, "malformed restart file " + name);
+    const int L = c->L;
+    const uint64_t mine = (uint64_t)((L + bit_length - 1) / bit_length);
+    const int64_t na = c->a.size(), nb = c->b.size();
+    // position of each loaded string in the current sets (find on equal word vectors)
+    auto place = [&](const std::vector<uint64_t>& wv, uint64_t n, const std::vector<uint64_t>& set) {
+      std::vector<int64_t> pos(n, -1);
+      if (words != mine) return pos;  // word vectors of another length never compare equal
+      std::unordered_map<uint64_t, int64_t> at;
+      for (size_t i = 0; i < set.size(); ++i) at.emplace(set[i], (int64_t)i);
+      for (uint64_t i = 0; i < n; ++i) {
+        uint64_t s;
+        if (!from_words(&wv[i * words], words, bit_length, s)) continue;
+        auto it = at.find(s);
+        if (it != at.end()) pos[i] = it->second;
+      }
+      return pos;
+    };
+    const std::vector<int64_t> pa = place(wa, la, c->a), pb = place(wb, lb, c->b);
+    std::vector<double> w((size_t)(na * nb), 0.0);
+    for (uint64_t i = 0; i < la; ++i) {
+      if (pa[i] < 0) continue;
+      for (uint64_t j = 0; j < lb; ++j)
+        if (pb[j] >= 0) w[(size_t)(pa[i] * nb + pb[j])] += lw[i * lb + j];
+    }
+    double nrm = 0.0;
+    for (double x : w) nrm += x * x;
+    nrm = std::sqrt(nrm);
+    if (!(nrm > 0.0)) return fail(c, SBDG_ERR_NUMERIC, "no determinant of " + name + " is in the current basis");
+    for (double& x : w) x /= nrm;
+    c->user_vec = std::move(w);
+    c->have_user_vec = true;
+    c->opt.init = 2;  // the loaded vector is the initial guess, as diag does with a loadname
+    return SBDG_OK;
+  });
+}
+
+int sbdg_save_matrix_form(sbdg_ctx* c, const char* filename, int bit_length) {
+  return guard(c, [&]() {
+    if (!c->solved) return fail(c, SBDG_ERR_STATE, "no solution");
+    if (!filename) return fail(c, SBDG_ERR_ARG, "null file name");
+    if (check_bit_length(c, bit_length)) return SBDG_ERR_ARG;
+    // SaveMatrixFormWF (restart.h:662-780): by extension
+    const std::string name(filename);
+    const size_t dot = name.find_last_of('.');
+    const std::string ext = dot == std::string::npos ? std::string() : name.substr(dot + 1);
+    if (ext != "bin" && ext != "txt" && ext != "dat") return SBDG_OK;  // nothing written
+    const std::vector<double> w = host_vector(c);
+    const int64_t na = c->a.size(), nb = c->b.size();
+    if (ext == "bin") {
+      std::ofstream o(name, std::ios::binary);
+      if (!o) return fail(c, SBDG_ERR_ARG, "cannot open " + name);
+      o.write(reinterpret_cast<const char*>(w.data()), (std::streamsize)(sizeof(double) * w.size()));
+      if (!o) return fail(c, SBDG_ERR_ARG, "write failed: " + name);
+      return SBDG_OK;
+    }
+    const int L = c->L;
+    auto str = [&](uint64_t s) {  // makestring (bit_manipulation.h:135-149): orbital L-1 first
+      std::string t((size_t)L, '0');
+      for (int i = 0; i < L; ++i)
+        if ((s >> i) & 1ull) t[(size_t)(L - 1 - i)] = '1';
+      return t;
+    };
+    std::ofstream o(name);  // default stream precision (six significant digits), as the reference
+    if (!o) return fail(c, SBDG_ERR_ARG, "cannot open " + name);
+    for (int64_t ia = 0; ia < na; ++ia) {
+      const std::string sa = str(c->a[ia]);
+      for (int64_t ib = 0; ib < nb; ++ib)
+        o << w[(size_t)(ia * nb + ib)] << " # " << ia << ": " << sa << " " << ib << ": " << str(c->b[ib]) << "\n";
+    }
+    if (!o) return fail(c, SBDG_ERR_ARG, "write failed: " + name);
     return SBDG_OK;
   });
 }

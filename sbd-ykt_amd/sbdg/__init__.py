@@ -60,6 +60,9 @@ _U64 = C.POINTER(C.c_uint64)
 # every entry point of include/sbdg.h with its ctypes signature (restype, argtypes)
 SIGNATURES = {
     "sbdg_create": (_P, [C.c_int]),
+    "sbdg_save_wavefunction": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "sbdg_load_wavefunction": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "sbdg_save_matrix_form": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "sbdg_destroy": (None, [_P]),
     "sbdg_last_error": (C.c_char_p, [_P]),
     "sbdg_default_options": (C.c_int, [C.POINTER(Options)]),
@@ -263,6 +266,19 @@ class Solver:
         out = np.zeros(self.ndet)
         self._chk(lib().sbdg_sigma(self._c, _d(c), _d(out), int(precision)))
         return out
+
+    def save_wavefunction(self, prefix, bit_length=20):
+        """diag's savename: writes <prefix>000000 in the reference's restart layout."""
+        self._chk(lib().sbdg_save_wavefunction(self._c, os.fsencode(prefix), int(bit_length)))
+
+    def load_wavefunction(self, prefix, bit_length=20):
+        """diag's loadname: <prefix>000000 remapped onto this basis becomes the initial vector."""
+        self._chk(lib().sbdg_load_wavefunction(self._c, os.fsencode(prefix), int(bit_length)))
+        self.options.init = 2
+
+    def save_matrix_form(self, filename, bit_length=20):
+        """dump_matrix_form_wf: .bin raw doubles, .txt/.dat one text line per determinant."""
+        self._chk(lib().sbdg_save_matrix_form(self._c, os.fsencode(filename), int(bit_length)))
 
     def bench_sigma(self, precision=0, reps=3):
         ms = np.zeros(6)

@@ -9,6 +9,10 @@ seeded recipe plus a checksum of them) next to the reference's outputs:
   ``sbd::tpb::diag`` (energy, 1-/2-RDM, density; sbdiag.h:113-477) at tol 1e-10.
 * ``ref_c0.npz`` -- the norb=12, 4+4 full-space workload (seed 0): diag at tol 1e-8 with
   RDMs, plus checksums of one sigma on a seeded vector (the vector itself is 2 MB).
+* ``ref_restart.npz`` -- the wavefunction files of diag (restart.h): the per-rank restart
+  file written by ``savename`` (bytes), the matrix-form dump in text and binary form, and
+  the vector obtained by loading that restart file into a different basis (``loadname``
+  with a tolerance so loose that Davidson stops at the loaded, normalised vector).
 * ``ref_n2_1em3.npz`` / ``ref_h2o_1em3.npz`` -- the reference datasets (data/n2, data/h2o)
   with the threshold-1e-3 string files: energy and 1-RDM from full-precision integrals,
   and the energy from integrals rounded to six digits (the published-table path).
@@ -137,6 +141,59 @@ def case_c0():
     print("c0", wl.ndet, res["energy"])
 
 
+def restart_bases():
+    """The asym_L8 workload (basis 1) and basis 2: five alpha and two beta strings
+    dropped, four alpha and three beta strings of the full space added."""
+    import itertools
+    wl = syn.Workload(*_sampled(8, 3, 2, 30, 20, 11))
+    L = wl.norb
+
+    def full(n):
+        return [sum(1 << o for o in c) for c in itertools.combinations(range(L), n)]
+
+    a1, b1 = [int(x) for x in wl.astr], [int(x) for x in wl.bstr]
+    a2 = [x for i, x in enumerate(a1) if i not in (3, 7, 11, 19, 25)]
+    a2 += [x for x in full(wl.nalpha) if x not in a1][:4]
+    b2 = [x for i, x in enumerate(b1) if i not in (2, 9)]
+    b2 += [x for x in full(wl.nbeta) if x not in b1][-3:]
+    return wl, np.array(sorted(a2), dtype=np.uint64), np.array(sorted(b2), dtype=np.uint64)
+
+
+def case_restart():
+    wl, a2, b2 = restart_bases()
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        L = wl.norb
+        fio.write_fcidump(os.path.join(tmp, "fcidump.txt"), L, wl.nalpha + wl.nbeta, wl.h1, wl.eri,
+                          core=wl.core, ms2=wl.nalpha - wl.nbeta)
+        fio.write_strings(os.path.join(tmp, "a.txt"), wl.astr, L)
+        fio.write_strings(os.path.join(tmp, "b.txt"), wl.bstr, L)
+        fio.write_strings(os.path.join(tmp, "a2.txt"), a2, L)
+        fio.write_strings(os.path.join(tmp, "b2.txt"), b2, L)
+        run_ref(["diag", "--fcidump", "fcidump.txt", "--adet", "a.txt", "--bdet", "b.txt", "--block", "10",
+                 "--iteration", "200", "--tolerance", "1e-10", "--rdm", "0", "--out", "d",
+                 "--save", "rs", "--dump_matrix_form_wf", "mf.txt"], tmp)
+        with open(os.path.join(tmp, "d.energy.txt")) as f:
+            e1 = float(f.readline())
+        rs = np.fromfile(os.path.join(tmp, "rs000000"), dtype=np.uint8)
+        with open(os.path.join(tmp, "mf.txt")) as f:
+            mf_txt = f.read()
+        run_ref(["diag", "--fcidump", "fcidump.txt", "--adet", "a.txt", "--bdet", "b.txt", "--block", "10",
+                 "--iteration", "200", "--tolerance", "1e-10", "--rdm", "0", "--out", "d"], tmp)
+        wf1 = rd(os.path.join(tmp, "d.wf.bin"))
+        run_ref(["diag", "--fcidump", "fcidump.txt", "--adet", "a2.txt", "--bdet", "b2.txt", "--block", "2",
+                 "--iteration", "1", "--tolerance", "1e10", "--rdm", "0", "--out", "l", "--load", "rs"], tmp)
+        with open(os.path.join(tmp, "l.energy.txt")) as f:
+            e_loaded = float(f.readline())
+        wf_loaded = rd(os.path.join(tmp, "l.wf.bin"))
+        res = dict(energy=e1, restart_bytes=rs, matrix_form_txt=np.array(mf_txt), wf=wf1, astr2=a2, bstr2=b2,
+                   wf_loaded=wf_loaded, energy_loaded=e_loaded, bit_length=20)
+        np.savez_compressed(os.path.join(HERE, "ref_restart.npz"), **res)
+        print("restart", e1, len(rs), e_loaded, wf_loaded.size)
+    finally:
+        shutil.rmtree(tmp)
+
+
 def case_dataset(tag, fcidump, adetfile, rounded_too=True):
     tmp = tempfile.mkdtemp(prefix="golden_")
     try:
@@ -184,7 +241,7 @@ def copy_datasets():
 
 
 if __name__ == "__main__":
-    want = sys.argv[1:] or ["small", "datasets", "c0"]
+    want = sys.argv[1:] or ["small", "datasets", "c0", "restart"]
     if "small" in want:
         case_small()
     if "datasets" in want:
@@ -193,3 +250,5 @@ if __name__ == "__main__":
         case_dataset("h2o_1em3", f"{REFDATA}/h2o/fcidump.txt", f"{REFDATA}/h2o/h2o-1em3-alpha.txt")
     if "c0" in want:
         case_c0()
+    if "restart" in want:
+        case_restart()
