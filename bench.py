@@ -195,17 +195,22 @@ def main():
         e64 = s.run()
         t64, tr64 = s.trajectory()
         st64 = s.stats()
-        s.set_options(precision=1, refresh_k=4, eps=1e-5)
-        emx = s.run()
-        tmx, trmx = s.trajectory()
-        stmx = s.stats()
         a = time_to_de(t64, tr64, e64)
-        b = time_to_de(tmx, trmx, e64)
-        tts = dict(e_fp64=e64, e_mixed=emx, mixed_error=abs(emx - e64), fp64_s=a, mixed_s=b,
-                   speedup=(a / b) if (a and b) else None, fp64_sigma=st64["sigma_fp64"],
-                   mixed_sigma_fp32=stmx["sigma_fp32"], mixed_sigma_fp64=stmx["sigma_fp64"],
-                   fp64_davidson_s=st64["t_davidson"], mixed_davidson_s=stmx["t_davidson"])
-        print(f"[bench] time-to-|dE|<1e-6: fp64 {a} s, mixed {b} s; mixed error {abs(emx - e64):.2e}", flush=True)
+        tts = dict(e_fp64=e64, fp64_s=a, fp64_sigma=st64["sigma_fp64"], fp64_davidson_s=st64["t_davidson"])
+        for k in (4, 8):
+            s.set_options(precision=1, refresh_k=k, eps=1e-5)
+            emx = s.run()
+            tmx, trmx = s.trajectory()
+            stmx = s.stats()
+            b = time_to_de(tmx, trmx, e64)
+            key = "" if k == 4 else f"_k{k}"
+            tts.update({f"e_mixed{key}": emx, f"mixed_error{key}": abs(emx - e64), f"mixed_s{key}": b,
+                        f"speedup{key}": (a / b) if (a and b) else None,
+                        f"mixed_sigma_fp32{key}": stmx["sigma_fp32"], f"mixed_sigma_fp64{key}": stmx["sigma_fp64"],
+                        f"mixed_davidson_s{key}": stmx["t_davidson"]})
+            print(f"[bench] time-to-|dE|<1e-6: fp64 {a} s, mixed (refresh every {k}) {b} s; "
+                  f"mixed error {abs(emx - e64):.2e}", flush=True)
+        tts["refresh_k"] = 4
     s.close()
 
     cpu = None if args.no_cpu else cpu_baseline(wl, args.cpu_rows)

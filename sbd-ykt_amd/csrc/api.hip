@@ -62,7 +62,7 @@ struct sbdg_ctx {
   DBuf<float> Wp32;
   DBuf<double> Zbuf;      // core-product workspace
   AbTable ab64;     // beta singles by chunk (RDMs)
-  EllTable ell64, ell32;  // beta singles, ELL (alpha-beta sigma)
+  EllTable ell;      // beta singles, ELL (alpha-beta sigma, FP64 and FP32 alike)
   // solver state
   DBuf<double> V, HV, W, R, T, HX, scr64, small;
   DBuf<float> HV32, c32, s32, scr32;
@@ -101,9 +101,10 @@ double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// column chunk of the alpha-beta kernel: two ket-row chunks and two L*L slices in LDS
-int64_t choose_chunk(int64_t nb, int L, int elem) {
-  const int64_t budget = (int64_t)(LDS_BUDGET / elem) / 2 - (int64_t)L * L - 1;
+// column chunk of the alpha-beta kernel: ket-row chunks and L*L slices of 16-byte row
+// vectors (2 FP64 or 4 FP32 rows) in LDS
+int64_t choose_chunk(int64_t nb, int L) {
+  const int64_t budget = (int64_t)(LDS_BUDGET / 16) - (int64_t)L * L - 1;
   int64_t ch = std::max<int64_t>(256, budget);
   return std::min<int64_t>(std::max<int64_t>(nb, 1), ch);
 }
@@ -118,7 +119,8 @@ SigmaPlan plan(sbdg_ctx* c, bool fp32) {
   P.ma = &c->Ma;
   P.mb = &c->Mb;
   P.sa = &c->Sa;
-  P.ell = fp32 ? &c->ell32 : &c->ell64;
+  P.ell = &c->ell;
+  (void)fp32;
   P.eri = c->d_eri.p;
   P.eri32 = c->d_eri32.p;
   P.hii = c->d_hii.p;
@@ -184,8 +186,7 @@ void build(sbdg_ctx* c) {
   }
   build_ab(c->Sb, L, std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - (int64_t)L * L)),
            c->ab64, c->st);
-  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 8), 8, c->ell64, c->st);
-  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L, 4), 4, c->ell32, c->st);
+  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L), 16, c->ell, c->st);  // both precisions
   c->d_hii.alloc(std::max<int64_t>(N, 1));
   c->d_hii32.alloc(std::max<int64_t>(N, 1));
   compute_hii(c->d_a.p, na, c->d_b.p, nb, L, c->core, c->d_h1.p, c->d_eri.p, c->d_hii.p, c->st);

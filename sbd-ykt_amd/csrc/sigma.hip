@@ -144,10 +144,15 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
 // for the members x of one product block (up to CORE_XB members of one core).  Members
 // whose pairs overlap have W = 0 (they differ by a single excitation, or are equal), so
 // every double pair of the string set is counted exactly once, in the core it shares.
-// The block's coefficients are wave-uniform (scalar loads); each lane owns VN columns.
-// workgroup = (group, column tile), tiles fastest (consecutive workgroups = consecutive tiles, so a
-// tile stays on one XCD for every group when the tile count is a multiple of 8), then
-// block groups.
+// The block's signed coefficients and member rows are staged in LDS, CORE_YC members at a
+// time (every lane reads the same entry: broadcast), and four member rows are loaded
+// before they are used.  Each lane owns VN columns.
+// Workgroups are dealt to the 8 XCDs in turn; workgroup L takes work item
+// w = (L % 8) * per + L / 8 (per = ceil(items / 8)), so each XCD walks one contiguous range of
+// items, and item w is column tile w / ngroups, block group w % ngroups: the blocks of one
+// core are neighbours in that range and find the core's member rows in their XCD's L2.
+constexpr int CORE_YC = 128;  // members staged per step
+
 template <class T, bool VECTOR>
 __global__ void __launch_bounds__(TS) k_core_prod(int64_t ncols, int64_t cb, int64_t ce,
                                                   const int32_t* __restrict__ start,
@@ -155,16 +160,23 @@ __global__ void __launch_bounds__(TS) k_core_prod(int64_t ncols, int64_t cb, int
                                                   const int32_t* __restrict__ pair,
                                                   const int32_t* __restrict__ blk,
                                                   const int32_t* __restrict__ blk_x0,
-                                                  const int32_t* __restrict__ grp, const T* __restrict__ W,
-                                                  int64_t npair, const T* __restrict__ X, T* __restrict__ Z,
-                                                  int64_t zcols) {
+                                                  const int32_t* __restrict__ grp, int64_t ngroups,
+                                                  const T* __restrict__ W, int64_t npair, const T* __restrict__ X,
+                                                  T* __restrict__ Z, int64_t zcols) {
   constexpr int VN = V16<T>::n;
   constexpr int XB = CORE_XB;
   using VT = typename V16<T>::t;
+  __shared__ __align__(16) T cf[CORE_YC][XB];  // signed coefficients of the staged members
+  __shared__ int32_t rw[CORE_YC];              // their rows
   const int64_t ntiles = (ce - cb + TS * VN - 1) / (TS * VN);
-  const int64_t g = blockIdx.x / ntiles;
-  const int64_t c0 = cb + ((int64_t)blockIdx.x - g * ntiles) * (TS * VN) + (int64_t)threadIdx.x * VN;
-  if (c0 >= ce) return;
+  const int64_t ngrp = ngroups;
+  const int64_t per = (ntiles * ngrp + 7) / 8;
+  const int64_t w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (w >= ntiles * ngrp) return;  // whole workgroup: no barrier is skipped by part of it
+  const int64_t tile = w / ngrp;
+  const int64_t g = w - tile * ngrp;
+  const int64_t c0 = cb + tile * (TS * VN) + (int64_t)threadIdx.x * VN;
+  const bool active = c0 < ce;  // inactive lanes still take part in the barriers
   const bool full = VECTOR && (c0 + VN <= ce);
   const int64_t zc = c0 - cb;
   for (int b = grp[g]; b < grp[g + 1]; ++b) {
@@ -173,48 +185,79 @@ __global__ void __launch_bounds__(TS) k_core_prod(int64_t ncols, int64_t cb, int
     const int m = start[core + 1] - s0;
     const int x0 = blk_x0[b];
     const int nx = min(XB, m - x0);
-    const T* wr[XB];
-    bool nxg[XB];
-#pragma unroll
-    for (int xi = 0; xi < XB; ++xi) {
-      const int pr = xi < nx ? pair[s0 + x0 + xi] : 0;
-      wr[xi] = W + (int64_t)(pr & 0x7FFFFFFF) * npair;
-      nxg[xi] = pr < 0;
-    }
     T acc[XB][VN];
 #pragma unroll
     for (int xi = 0; xi < XB; ++xi)
 #pragma unroll
       for (int k = 0; k < VN; ++k) acc[xi][k] = T(0);
-    for (int y = 0; y < m; ++y) {
-      const int pr = pair[s0 + y];
-      const int py = pr & 0x7FFFFFFF;
-      const bool ny = pr < 0;
-      const T* x = X + (int64_t)row[s0 + y] * ncols + c0;
-      T xv[VN];
-      if (full) {
-        *reinterpret_cast<VT*>(xv) = *reinterpret_cast<const VT*>(x);
-      } else {
-#pragma unroll
-        for (int k = 0; k < VN; ++k) xv[k] = (c0 + k < ce) ? x[k] : T(0);
+    for (int y0 = 0; y0 < m; y0 += CORE_YC) {
+      const int ny = min(CORE_YC, m - y0);
+      __syncthreads();  // previous step done with the staged entries
+      for (int e = threadIdx.x; e < ny * XB; e += TS) {
+        const int yy = e / XB, xi = e - yy * XB;
+        T v = T(0);
+        if (xi < nx) {
+          const int px = pair[s0 + x0 + xi], py = pair[s0 + y0 + yy];
+          v = W[(int64_t)(px & 0x7FFFFFFF) * npair + (py & 0x7FFFFFFF)];
+          if ((px < 0) != (py < 0)) v = -v;
+        }
+        cf[yy][xi] = v;
+        if (xi == 0) rw[yy] = row[s0 + y0 + yy];
       }
+      __syncthreads();
+      if (active) {
+        int yy = 0;
+        for (; yy + 4 <= ny; yy += 4) {
+          T xv[4][VN];
 #pragma unroll
-      for (int xi = 0; xi < XB; ++xi) {
-        const T w = (nxg[xi] != ny) ? -wr[xi][py] : wr[xi][py];
+          for (int u = 0; u < 4; ++u) {
+            const T* x = X + (int64_t)rw[yy + u] * ncols + c0;
+            if (full) {
+              *reinterpret_cast<VT*>(xv[u]) = *reinterpret_cast<const VT*>(x);
+            } else {
 #pragma unroll
-        for (int k = 0; k < VN; ++k) acc[xi][k] += w * xv[k];
+              for (int k = 0; k < VN; ++k) xv[u][k] = (c0 + k < ce) ? x[k] : T(0);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int xi = 0; xi < XB; ++xi) {
+              const T w = cf[yy + u][xi];
+#pragma unroll
+              for (int k = 0; k < VN; ++k) acc[xi][k] += w * xv[u][k];
+            }
+        }
+        for (; yy < ny; ++yy) {
+          T xv[VN];
+          const T* x = X + (int64_t)rw[yy] * ncols + c0;
+          if (full) {
+            *reinterpret_cast<VT*>(xv) = *reinterpret_cast<const VT*>(x);
+          } else {
+#pragma unroll
+            for (int k = 0; k < VN; ++k) xv[k] = (c0 + k < ce) ? x[k] : T(0);
+          }
+#pragma unroll
+          for (int xi = 0; xi < XB; ++xi) {
+            const T w = cf[yy][xi];
+#pragma unroll
+            for (int k = 0; k < VN; ++k) acc[xi][k] += w * xv[k];
+          }
+        }
       }
     }
+    if (active) {
 #pragma unroll
-    for (int xi = 0; xi < XB; ++xi) {
-      if (xi < nx) {
-        T* z = Z + (int64_t)(s0 + x0 + xi) * zcols + zc;
-        if (full) {
-          *reinterpret_cast<VT*>(z) = *reinterpret_cast<VT*>(acc[xi]);
-        } else {
+      for (int xi = 0; xi < XB; ++xi) {
+        if (xi < nx) {
+          T* z = Z + (int64_t)(s0 + x0 + xi) * zcols + zc;
+          if (full) {
+            *reinterpret_cast<VT*>(z) = *reinterpret_cast<VT*>(acc[xi]);
+          } else {
 #pragma unroll
-          for (int k = 0; k < VN; ++k)
-            if (c0 + k < ce) z[k] = acc[xi][k];
+            for (int k = 0; k < VN; ++k)
+              if (c0 + k < ce) z[k] = acc[xi][k];
+          }
         }
       }
     }
@@ -247,32 +290,30 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
 // alpha-beta and cross-spin single terms.  One workgroup per (bra row ia, tile of lane
 // positions); thread t owns positions col0 + t + m*blockDim.x, m < M (every wave owns
 // whole 64-lane groups of the ELL table), i.e. columns colperm[position].  The ket rows
-// ja in {ia} + S(ia) are taken two at a time: both (ai|..) slices (for ia itself
-// sum_{k in occ(ia)} (kk|..) with a zeroed diagonal) and both rows' current column chunk
-// sit in LDS interleaved (one 2-element LDS read serves both rows), and every ELL entry
-// read from the L2-resident table serves both rows.  Entry lists are padded to multiples
-// of 4 and read four at a time.  LDS: [(chunk + 1 zero slot) pairs] [L*L slice pairs].
+// ja in {ia} + S(ia) are taken G at a time (G = 2 in FP64, 4 in FP32: 16 bytes either
+// way): the G (ai|..) slices (for ia itself sum_{k in occ(ia)} (kk|..) with a zeroed
+// diagonal) and the G rows' current column chunk sit in LDS interleaved, so one 16-byte
+// LDS read serves all G rows, and every ELL entry read from the L2-resident table serves
+// them too.  LDS: [(chunk + 1 zero slot) G-vectors] [L*L slice G-vectors].
 template <class T>
-struct Pair2;
-template <>
-struct Pair2<double> {
-  using t = double2;
-  __device__ static inline double flip(double v, uint32_t neg) {
-    return __longlong_as_double(__double_as_longlong(v) ^ ((unsigned long long)neg << 63));
-  }
-};
-template <>
-struct Pair2<float> {
-  using t = float2;
-  __device__ static inline float flip(float v, uint32_t neg) {
-    return __int_as_float(__float_as_int(v) ^ (int)(neg << 31));
+struct AbRows {
+  static constexpr int G = 16 / (int)sizeof(T);
+  struct __align__(16) V {
+    T v[G];
+  };
+  __device__ static inline T flip(T x, uint32_t neg) {
+    if constexpr (sizeof(T) == 8)
+      return __longlong_as_double(__double_as_longlong(x) ^ ((unsigned long long)neg << 63));
+    else
+      return __int_as_float(__float_as_int(x) ^ (int)(neg << 31));
   }
 };
 
 template <class T>
 __device__ inline void ab_slice(bool self, uint64_t A, int ai, int L, int64_t LL, const T* __restrict__ eri,
-                                T* __restrict__ V, int lane_of_pair) {
-  // V points at the pair array; element x of this row lives at V[2*x + lane_of_pair]
+                                T* __restrict__ V, int g) {
+  // V is the G-vector array; element x of row g lives at V[G*x + g]
+  constexpr int G = AbRows<T>::G;
   if (self) {
     for (int x = threadIdx.x; x < LL; x += blockDim.x) {
       T v = T(0);
@@ -284,21 +325,23 @@ __device__ inline void ab_slice(bool self, uint64_t A, int ai, int L, int64_t LL
           v += eri[((int64_t)k * L + k) * LL + x];
         }
       }
-      V[2 * x + lane_of_pair] = v;
+      V[G * x + g] = v;
     }
   } else {
     const T* src = eri + (int64_t)ai * LL;
-    for (int x = threadIdx.x; x < LL; x += blockDim.x) V[2 * x + lane_of_pair] = src[x];
+    for (int x = threadIdx.x; x < LL; x += blockDim.x) V[G * x + g] = src[x];
   }
 }
 
-// Entries of one lane in one (group, chunk): sum_e sign_e * V[pair_e] * cb[jb_e] for the two
-// staged rows at once.  n is wave-uniform; four entries are in flight while the previous
-// four are used.
+// Entries of one lane in one (group, chunk): s[g] += sign_e * V[pair_e][g] * cb[jb_e][g] for
+// the G staged rows at once.  n is wave-uniform; four entries are in flight while the
+// previous four are used.
 template <class T>
-__device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const typename Pair2<T>::t* cbp,
-                                  const typename Pair2<T>::t* Vp, T& s0, T& s1) {
-  using T2 = typename Pair2<T>::t;
+__device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const typename AbRows<T>::V* cbv,
+                                  const typename AbRows<T>::V* Vv, T (&s)[AbRows<T>::G]) {
+  using R = AbRows<T>;
+  using RV = typename R::V;
+  constexpr int G = R::G;
   int e = 0;
   if (n >= 4) {
     uint32_t pn[4];
@@ -314,21 +357,21 @@ __device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const 
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const T2 v = Vp[(p[u] >> 18) & 0xFFFu];
-        const T2 x = cbp[p[u] & 0x3FFFFu];
+        const RV v = Vv[(p[u] >> 18) & 0xFFFu];
+        const RV x = cbv[p[u] & 0x3FFFFu];
         const uint32_t neg = (p[u] >> 30) & 1u;
-        s0 += Pair2<T>::flip(v.x, neg) * x.x;
-        s1 += Pair2<T>::flip(v.y, neg) * x.y;
+#pragma unroll
+        for (int g = 0; g < G; ++g) s[g] += R::flip(v.v[g], neg) * x.v[g];
       }
     }
   }
   for (; e < n; ++e) {
     const uint32_t p = ep[(uint32_t)e * 64u];
-    const T2 v = Vp[(p >> 18) & 0xFFFu];
-    const T2 x = cbp[p & 0x3FFFFu];
+    const RV v = Vv[(p >> 18) & 0xFFFu];
+    const RV x = cbv[p & 0x3FFFFu];
     const uint32_t neg = (p >> 30) & 1u;
-    s0 += Pair2<T>::flip(v.x, neg) * x.x;
-    s1 += Pair2<T>::flip(v.y, neg) * x.y;
+#pragma unroll
+    for (int g = 0; g < G; ++g) s[g] += R::flip(v.v[g], neg) * x.v[g];
   }
 }
 
@@ -336,10 +379,11 @@ __device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const 
 //   sigma[ia, ib] += sum_{ja in S(ia)+{ia}} coef * sum_{entries (jb, b<-j) of ib} sb V_ja[b*L+j] c[ja, jb]
 // with V_ja the (ai|..) slice of the alpha single ja (coef its sign) or, for ja = ia, the
 // alpha-occupation slice sum_{k in occ(ia)} (kk|..) (off-diagonal part).  Chunk by chunk of
-// ket columns: two ket rows of the chunk and their slices sit in LDS as interleaved pairs,
-// each lane walks its ELL entries for that chunk (the lane -> column map is per chunk, so
-// lanes of a group carry lists of similar length), and the chunk's sums are added into
-// sigma before the next chunk (one owner per element: a column belongs to one tile).
+// ket columns: G ket rows of the chunk and their slices sit in LDS as interleaved
+// G-vectors, each lane walks its ELL entries for that chunk (the lane -> column map is per
+// chunk, so lanes of a group carry lists of similar length), and the chunk's sums are
+// added into sigma before the next chunk (one owner per element: a column belongs to one
+// tile).
 template <class T, int M>
 __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, const uint64_t* __restrict__ astr,
                                              const int64_t* __restrict__ rpSa,
@@ -350,11 +394,13 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
                                              const int32_t* __restrict__ colperm, int nch, int64_t chunk,
                                              int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
                                              T* __restrict__ sigma) {
-  using T2 = typename Pair2<T>::t;
+  using R = AbRows<T>;
+  using RV = typename R::V;
+  constexpr int G = R::G;
   extern __shared__ __align__(16) unsigned char smem[];
-  T2* cbp = reinterpret_cast<T2*>(smem);  // chunk + 1 pairs (the last one stays zero: padding)
-  T2* Vp = cbp + chunk + 1;               // L*L pairs
-  T* Vs = reinterpret_cast<T*>(Vp);
+  RV* cbv = reinterpret_cast<RV*>(smem);  // chunk + 1 G-vectors (the last one stays zero: padding)
+  RV* Vv = cbv + chunk + 1;               // L*L G-vectors
+  T* Vs = reinterpret_cast<T*>(Vv);
   const int64_t LL = (int64_t)L * L;
   const int NT = blockDim.x;
   const int64_t npos = groups * 64;
@@ -362,7 +408,8 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
   const int64_t col0 = (blockIdx.x - ia * ntiles) * (int64_t)NT * M;
   const int lane = threadIdx.x & 63;
   const int64_t gw = (col0 >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  if (threadIdx.x == 0) cbp[chunk] = T2{T(0), T(0)};
+  if (threadIdx.x == 0)
+    for (int g = 0; g < G; ++g) cbv[chunk].v[g] = T(0);
   const int64_t q0 = rpSa[ia], q1 = rpSa[ia + 1];
   const uint64_t A = astr[ia];
   for (int k = 0; k < nch; ++k) {
@@ -371,46 +418,46 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
     T acc[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) acc[m] = T(0);
-    for (int64_t t = q0 - 1; t < q1; t += 2) {
-      const bool self = (t < q0);
-      const bool two = (t + 1 < q1);
-      int64_t ja0 = ia, ja1 = ia;
-      T coef0 = T(1), coef1 = T(0);
-      int ai0 = 0, ai1 = 0;
-      if (!self) {
-        ja0 = colSa[t];
-        const uint32_t e = excSa[t];
-        ai0 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);  // a*L + i
-        coef0 = ((e >> 24) & 1u) ? T(-1) : T(1);
+    for (int64_t t = q0 - 1; t < q1; t += G) {
+      // rows t..t+G-1: t = q0-1 is ia itself; rows at or past q1 are absent (zero)
+      T coef[G];
+      const T* crow[G];
+      bool has[G];
+      __syncthreads();  // previous rows done with LDS (and the previous chunk's sums are stored)
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int64_t q = t + g;
+        has[g] = q < q1;
+        coef[g] = T(0);
+        crow[g] = c + ia * nb + j0;
+        if (q < q0) {
+          coef[g] = T(1);
+          ab_slice<T>(true, A, 0, L, LL, eri, Vs, g);
+        } else if (has[g]) {
+          const uint32_t e = excSa[q];
+          coef[g] = ((e >> 24) & 1u) ? T(-1) : T(1);
+          crow[g] = c + (int64_t)colSa[q] * nb + j0;
+          ab_slice<T>(false, A, (int)((e >> 6) & 63u) * L + (int)(e & 63u), L, LL, eri, Vs, g);  // a*L + i
+        } else {
+          for (int x = threadIdx.x; x < LL; x += NT) Vs[G * x + g] = T(0);
+        }
       }
-      if (two) {
-        ja1 = colSa[t + 1];
-        const uint32_t e = excSa[t + 1];
-        ai1 = (int)((e >> 6) & 63u) * L + (int)(e & 63u);
-        coef1 = ((e >> 24) & 1u) ? T(-1) : T(1);
-      }
-      __syncthreads();  // previous pair done with LDS (and the previous chunk's sums are stored)
-      ab_slice<T>(self, A, ai0, L, LL, eri, Vs, 0);
-      if (two) ab_slice<T>(false, A, ai1, L, LL, eri, Vs, 1);
-      else
-        for (int x = threadIdx.x; x < LL; x += NT) Vs[2 * x + 1] = T(0);
-      const T* crow0 = c + ja0 * nb + j0;
-      const T* crow1 = c + ja1 * nb + j0;
       for (int64_t x0 = threadIdx.x; x0 < nj; x0 += 4 * NT) {
-        T2 v[4];
+        RV v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int64_t x = x0 + (int64_t)u * NT;
-          v[u].x = x < nj ? __builtin_nontemporal_load(crow0 + x) : T(0);
-          v[u].y = (two && x < nj) ? __builtin_nontemporal_load(crow1 + x) : T(0);
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            v[u].v[g] = (has[g] && x < nj) ? __builtin_nontemporal_load(crow[g] + x) : T(0);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-          if (x0 + (int64_t)u * NT < nj) cbp[x0 + (int64_t)u * NT] = v[u];
+          if (x0 + (int64_t)u * NT < nj) cbv[x0 + (int64_t)u * NT] = v[u];
       }
       __syncthreads();
       // the wave's group index, opaque to the compiler here: the per-group offsets and
-      // counts are then reloaded for every pair instead of being held in 3M registers
+      // counts are then reloaded for every step instead of being held in 3M registers
       int gws = (int)gw;
       asm volatile("" : "+s"(gws));
 #pragma unroll
@@ -420,9 +467,14 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
           const int64_t idx = g * nch + k;
           const int64_t o = eoff[idx];
           const int n = eK[idx];
-          T s0 = T(0), s1 = T(0);
-          ab_entries<T>(ent + o + lane, n, cbp, Vp, s0, s1);
-          acc[m] += coef0 * s0 + coef1 * s1;
+          T sg[G];
+#pragma unroll
+          for (int r = 0; r < G; ++r) sg[r] = T(0);
+          ab_entries<T>(ent + o + lane, n, cbv, Vv, sg);
+          T a = acc[m];
+#pragma unroll
+          for (int r = 0; r < G; ++r) a += coef[r] * sg[r];
+          acc[m] = a;
         }
       }
     }
@@ -505,14 +557,16 @@ void launch_spin(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, c
     const int64_t ce = std::min(ncols, cb + zcols);
     if (ct->ngrp > 0) {
       const int64_t ntiles = (ce - cb + TS * VN - 1) / (TS * VN);
-      if (ntiles * ct->ngrp > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
-      dim3 g((unsigned)(ntiles * ct->ngrp)), b(TS);
+      const int64_t items = ntiles * ct->ngrp;
+      const int64_t nwg = (items + 7) / 8 * 8;
+      if (nwg > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
+      dim3 g((unsigned)nwg), b(TS);
       if (vec)
         hipLaunchKernelGGL((k_core_prod<T, true>), g, b, 0, st, ncols, cb, ce, ct->start.p, ct->row.p, ct->pair.p,
-                           ct->blk.p, ct->blk_x0.p, ct->grp.p, W, npair, X, Z, zcols);
+                           ct->blk.p, ct->blk_x0.p, ct->grp.p, ct->ngrp, W, npair, X, Z, zcols);
       else
         hipLaunchKernelGGL((k_core_prod<T, false>), g, b, 0, st, ncols, cb, ce, ct->start.p, ct->row.p,
-                           ct->pair.p, ct->blk.p, ct->blk_x0.p, ct->grp.p, W, npair, X, Z, zcols);
+                           ct->pair.p, ct->blk.p, ct->blk_x0.p, ct->grp.p, ct->ngrp, W, npair, X, Z, zcols);
     }
     launch_spmm<T>(diag, nrows, ncols, tb, X, hii, out, st, cb, ce, ct, Z, zcols);
   }
@@ -562,7 +616,8 @@ void transpose_d(int64_t R, int64_t C, const double* in, double* out, hipStream_
 }
 
 size_t ab_lds_bytes(int L, int64_t chunk, int elem) {
-  return (size_t)elem * 2 * ((size_t)chunk + 1 + (size_t)L * L);
+  return (size_t)16 * ((size_t)chunk + 1 + (size_t)L * L);  // 16-byte row vectors, any precision
+  (void)elem;
 }
 
 template <class T>

@@ -589,9 +589,9 @@ void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int e
   const int64_t total = off[ng];
   const uint32_t pad = (uint32_t)chunk;  // zero slot, pair 0
   std::vector<uint32_t> ent((size_t)std::max<int64_t>(total, 1), pad);
-  // LDS pass groups of the kernel's pair reads: 16-byte pairs (FP64) in groups of 8 lanes
-  // over 8 slots, 8-byte pairs (FP32) in groups of 16 lanes over 16 slots
-  const int P = elem == 8 ? 8 : 16, S = P;
+  // LDS pass groups of the kernel's row-vector reads: 16-byte vectors in groups of 8 lanes
+  // over 8 slots, 8-byte vectors in groups of 16 lanes over 16 slots
+  const int P = elem >= 16 ? 8 : 16, S = P;
   const uint32_t vbase = (uint32_t)(chunk + 1);  // integral pairs follow the ket pairs
   std::vector<std::vector<uint32_t>> lists(64);
   for (int64_t g = 0; g < e.groups; ++g)
@@ -656,18 +656,38 @@ void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct
     SBDG_CHECK(hipMemcpyAsync(hk.data(), key2.p, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, st));
     SBDG_CHECK(hipMemcpyAsync(hv.data(), val2.p, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, st));
     SBDG_CHECK(hipStreamSynchronize(st));
-    // keep cores with >= 2 members (radix sort is stable: members ascend by string)
+    // keep, core by core, the members whose pair is disjoint from some other member's pair:
+    // any other member is a single excitation away (or the same string), contributes
+    // nothing and receives nothing.  Radix sort is stable: members ascend by string.
+    auto pair_mask = [](uint64_t v) {
+      const int P = (int)((v >> 32) & 0x7FFFFFFFull);
+      int b = 1;
+      while ((b + 1) * b / 2 <= P) ++b;
+      const int a = P - b * (b - 1) / 2;
+      return (1ull << a) | (1ull << b);
+    };
+    std::vector<uint64_t> masks;
+    std::vector<int64_t> keep;
     for (int64_t e = 0; e < total;) {
       int64_t f = e + 1;
       while (f < total && hk[f] == hk[e]) ++f;
-      const int64_t m = f - e;
+      masks.clear();
+      keep.clear();
+      for (int64_t q = e; q < f; ++q) masks.push_back(pair_mask(hv[q]));
+      for (int64_t x = 0; x < f - e; ++x)
+        for (int64_t y = 0; y < f - e; ++y)
+          if ((masks[x] & masks[y]) == 0) {
+            keep.push_back(e + x);
+            break;
+          }
+      const int64_t m = (int64_t)keep.size();
       if (m >= 2) {
         const int32_t c = (int32_t)(start.size() - 1);
         for (int64_t x0 = 0; x0 < m; x0 += CORE_XB) {
           blk.push_back(c);
           blk_x0.push_back((int32_t)x0);
         }
-        for (int64_t q = e; q < f; ++q) {
+        for (int64_t q : keep) {
           row.push_back((int32_t)(hv[q] & 0xFFFFFFFFull));
           pair.push_back((int32_t)((hv[q] >> 32) & 0x7FFFFFFFull) | (int32_t)((hv[q] >> 63) << 31));
         }
