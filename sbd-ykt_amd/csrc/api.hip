@@ -58,6 +58,8 @@ struct sbdg_ctx {
   SpmmTable Ma, Mb;
   CoreTable Ca, Cb;       // same-spin doubles by core (used when use_ca / use_cb)
   bool use_ca = false, use_cb = false;
+  DBuf<double> Ja, Jb;    // occupation tables of the alpha / beta strings
+  DBuf<float> Ja32, Jb32;
   DBuf<double> Wp;        // antisymmetrised pair integrals
   DBuf<float> Wp32;
   DBuf<double> Zbuf;      // core-product workspace
@@ -125,6 +127,10 @@ SigmaPlan plan(sbdg_ctx* c, bool fp32) {
   P.eri32 = c->d_eri32.p;
   P.hii = c->d_hii.p;
   P.hii32 = c->d_hii32.p;
+  P.Ja = c->Ja.p;
+  P.Ja32 = c->Ja32.p;
+  P.Jb = c->Jb.p;
+  P.Jb32 = c->Jb32.p;
   P.ca = c->use_ca ? &c->Ca : nullptr;
   P.cb = c->use_cb ? &c->Cb : nullptr;
   P.W = c->Wp.p;
@@ -164,8 +170,17 @@ void build(sbdg_ctx* c) {
     c->use_ca = c->opt.doubles == 2 || c->Da.nnz > 4 * c->Ca.nmem;
     c->use_cb = c->opt.doubles == 2 || c->Db.nnz > 4 * c->Cb.nmem;
   }
-  build_spmm(c->Sa, c->Da, c->Ma, c->st, !c->use_ca);
-  build_spmm(c->Sb, c->Db, c->Mb, c->st, !c->use_cb);
+  build_spmm(c->Sa, c->Da, c->Ma, L, c->st, !c->use_ca);
+  build_spmm(c->Sb, c->Db, c->Mb, L, c->st, !c->use_cb);
+  // occupation parts of the singles: beta occupations for the alpha product (columns of
+  // c), alpha occupations for the beta product (columns of c^T)
+  const int64_t LL1 = (int64_t)L * L + 1;
+  c->Jb.alloc(std::max<int64_t>(LL1 * nb, 1));
+  c->Jb32.alloc(std::max<int64_t>(LL1 * nb, 1));
+  c->Ja.alloc(std::max<int64_t>(LL1 * na, 1));
+  c->Ja32.alloc(std::max<int64_t>(LL1 * na, 1));
+  build_occ_table(c->d_b.p, nb, L, c->d_eri.p, c->Jb.p, c->Jb32.p, c->st);
+  build_occ_table(c->d_a.p, na, L, c->d_eri.p, c->Ja.p, c->Ja32.p, c->st);
   if (c->use_ca || c->use_cb) {
     const int64_t npair = (int64_t)L * (L - 1) / 2;
     c->Wp.alloc(std::max<int64_t>(npair * npair, 1));
@@ -186,7 +201,7 @@ void build(sbdg_ctx* c) {
   }
   build_ab(c->Sb, L, std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - (int64_t)L * L)),
            c->ab64, c->st);
-  build_ell(c->Sb, c->d_b.p, L, choose_chunk(nb, L), 16, c->ell, c->st);  // both precisions
+  build_ell(c->Sb, L, choose_chunk(nb, L), 16, c->ell, c->st);  // both precisions
   c->d_hii.alloc(std::max<int64_t>(N, 1));
   c->d_hii32.alloc(std::max<int64_t>(N, 1));
   compute_hii(c->d_a.p, na, c->d_b.p, nb, L, c->core, c->d_h1.p, c->d_eri.p, c->d_hii.p, c->st);
@@ -697,7 +712,7 @@ int sbdg_set_strings(sbdg_ctx* c, const uint64_t* a, int64_t na, const uint64_t*
     std::string m = check_set(a, na, c->L, "alpha");
     if (m.empty()) m = check_set(b, nb, c->L, "beta");
     if (!m.empty()) return fail(c, SBDG_ERR_ARG, m);
-    if (nb > 65536) return fail(c, SBDG_ERR_LIMIT, "beta string set beyond 65536 strings");
+    if (nb > (1 << 18)) return fail(c, SBDG_ERR_LIMIT, "beta string set beyond 262144 strings");
     if (na > (int64_t)INT32_MAX) return fail(c, SBDG_ERR_LIMIT, "alpha string set beyond 2^31 strings");
     c->a.assign(a, a + na);
     c->b.assign(b, b + nb);

@@ -80,6 +80,10 @@ struct SpmmTable {
   DBuf<int32_t> col;
   DBuf<double> val;
   DBuf<float> val32;
+  // other-spin occupation term of a single: +-(p + 1) with p = a*L + i its orbital pair
+  // and the sign of the single; 0 for a double.  The product adds sign * J[p + 1][c] per
+  // column (J: occupation table of the column strings, row 0 zero).
+  DBuf<int32_t> jai;
 };
 
 // Packed beta singles for the alpha-beta kernel: bits 0-17 ket index jb, 18-29 orbital
@@ -138,12 +142,16 @@ constexpr int CORE_WORK = 256;  // member rows streamed per core-product workgro
 // kernels (tables.hip)
 void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, const double* d_eri,
                  Table& t_single, Table& t_double, hipStream_t st);
-void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st, bool with_doubles = true);
+void build_spmm(const Table& s, const Table& d, SpmmTable& m, int L, hipStream_t st, bool with_doubles = true);
 void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct, hipStream_t st);
+// J[(p+1)*n + r] = sum_{o in occ(s[r])} eri[p*L^2 + o*L + o] for p < L^2 (row 0 zero):
+// the occupation part of the other spin's singles, (ai|oo) summed over the string's orbitals
+void build_occ_table(const uint64_t* d_str, int64_t n, int L, const double* d_eri, double* J, float* J32,
+                     hipStream_t st);
 // W[P*npair + Q] (FP64 and FP32), npair = L(L-1)/2
 void build_pair_integrals(const double* d_eri, int L, double* W, float* W32, hipStream_t st);
 void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st);
-void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st);
+void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st);
 void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
                  const double* d_h1, const double* d_eri, double* d_hii, hipStream_t st);
 
@@ -161,6 +169,10 @@ struct SigmaPlan {
   const float* eri32 = nullptr;
   const double* hii = nullptr;
   const float* hii32 = nullptr;
+  const double* Jb = nullptr;  // beta-occupation table (columns of c), alpha singles
+  const float* Jb32 = nullptr;
+  const double* Ja = nullptr;  // alpha-occupation table (columns of c^T), beta singles
+  const float* Ja32 = nullptr;
   const CoreTable* ca = nullptr;  // alpha doubles by core (null: doubles in ma/mb)
   const CoreTable* cb = nullptr;
   const double* W = nullptr;

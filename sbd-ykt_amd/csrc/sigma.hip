@@ -36,13 +36,16 @@ struct V16<float> {
 constexpr int TS = 256;  // threads of the row-streaming product
 
 // Column range [cb, ce) of one row product:
-//   out[r, c] = (DIAG ? hii*X : 0)[r, c] + sum_q val[q] X[col[q], c]
+//   out[r, c] = (DIAG ? hii*X : 0)[r, c] + sum_q (val[q] + sgn_q J[p_q][c]) X[col[q], c]
 //             + (ZG ? sum_{s in slots(r)} Z[slot[s], c - cb] : 0)
-// Z holds the core-grouped doubles of the same column range (k_core_prod).
+// jai[q] = +-(p_q) selects the row of the column strings' occupation table J that carries
+// the other-spin occupation part of a single (row 0: zero, for doubles).  Z holds the
+// core-grouped doubles of the same column range (k_core_prod).
 template <class T, bool DIAG, bool VECTOR, bool ZG>
 __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64_t cb, int64_t ce,
                                              const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                             const T* __restrict__ val, const T* __restrict__ X,
+                                             const T* __restrict__ val, const int32_t* __restrict__ jai,
+                                             const T* __restrict__ J, const T* __restrict__ X,
                                              const T* __restrict__ hii, T* __restrict__ out,
                                              const int32_t* __restrict__ zptr, const int32_t* __restrict__ zslot,
                                              const T* __restrict__ Z, int64_t zcols) {
@@ -76,27 +79,31 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
   const int64_t q1 = rowptr[r + 1];
   if (full) {
     for (; q + 4 <= q1; q += 4) {
-      const int32_t j0 = col[q], j1 = col[q + 1], j2 = col[q + 2], j3 = col[q + 3];
-      const T v0 = val[q], v1 = val[q + 1], v2 = val[q + 2], v3 = val[q + 3];
-      T x0[VN], x1[VN], x2[VN], x3[VN];
-      *reinterpret_cast<VT*>(x0) = *reinterpret_cast<const VT*>(X + (int64_t)j0 * ncols + c0);
-      *reinterpret_cast<VT*>(x1) = *reinterpret_cast<const VT*>(X + (int64_t)j1 * ncols + c0);
-      *reinterpret_cast<VT*>(x2) = *reinterpret_cast<const VT*>(X + (int64_t)j2 * ncols + c0);
-      *reinterpret_cast<VT*>(x3) = *reinterpret_cast<const VT*>(X + (int64_t)j3 * ncols + c0);
+      int32_t j[4];
+      T v[4], sg[4], x[4][VN], jv[4][VN];
 #pragma unroll
-      for (int k = 0; k < VN; ++k) {
-        acc[k] += v0 * x0[k];
-        acc[k] += v1 * x1[k];
-        acc[k] += v2 * x2[k];
-        acc[k] += v3 * x3[k];
+      for (int u = 0; u < 4; ++u) {
+        j[u] = col[q + u];
+        v[u] = val[q + u];
+        const int32_t p = jai[q + u];
+        sg[u] = p < 0 ? T(-1) : T(1);
+        *reinterpret_cast<VT*>(x[u]) = *reinterpret_cast<const VT*>(X + (int64_t)j[u] * ncols + c0);
+        *reinterpret_cast<VT*>(jv[u]) = *reinterpret_cast<const VT*>(J + (int64_t)(p < 0 ? -p : p) * ncols + c0);
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc[k] += (v[u] + sg[u] * jv[u][k]) * x[u][k];
     }
     for (; q < q1; ++q) {
       const T v = val[q];
-      T x[VN];
+      const int32_t p = jai[q];
+      const T sg = p < 0 ? T(-1) : T(1);
+      T x[VN], jv[VN];
       *reinterpret_cast<VT*>(x) = *reinterpret_cast<const VT*>(X + (int64_t)col[q] * ncols + c0);
+      *reinterpret_cast<VT*>(jv) = *reinterpret_cast<const VT*>(J + (int64_t)(p < 0 ? -p : p) * ncols + c0);
 #pragma unroll
-      for (int k = 0; k < VN; ++k) acc[k] += v * x[k];
+      for (int k = 0; k < VN; ++k) acc[k] += (v + sg * jv[k]) * x[k];
     }
     if (ZG) {
       const int64_t zc = c0 - cb;
@@ -120,10 +127,13 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
   } else {
     for (; q < q1; ++q) {
       const T v = val[q];
+      const int32_t p = jai[q];
+      const T sg = p < 0 ? T(-1) : T(1);
       const T* x = X + (int64_t)col[q] * ncols + c0;
+      const T* jr = J + (int64_t)(p < 0 ? -p : p) * ncols + c0;
 #pragma unroll
       for (int k = 0; k < VN; ++k)
-        if (c0 + k < ce) acc[k] += v * x[k];
+        if (c0 + k < ce) acc[k] += (v + sg * jr[k]) * x[k];
     }
     if (ZG) {
       for (int32_t s = zptr[r]; s < zptr[r + 1]; ++s) {
@@ -290,11 +300,10 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
 // alpha-beta and cross-spin single terms.  One workgroup per (bra row ia, tile of lane
 // positions); thread t owns positions col0 + t + m*blockDim.x, m < M (every wave owns
 // whole 64-lane groups of the ELL table), i.e. columns colperm[position].  The ket rows
-// ja in {ia} + S(ia) are taken G at a time (G = 2 in FP64, 4 in FP32: 16 bytes either
-// way): the G (ai|..) slices (for ia itself sum_{k in occ(ia)} (kk|..) with a zeroed
-// diagonal) and the G rows' current column chunk sit in LDS interleaved, so one 16-byte
-// LDS read serves all G rows, and every ELL entry read from the L2-resident table serves
-// them too.  LDS: [(chunk + 1 zero slot) G-vectors] [L*L slice G-vectors].
+// The ja in S(ia) are taken G at a time (G = 2 in FP64, 4 in FP32: 16 bytes either way):
+// the G (ai|..) slices and the G rows' current column chunk sit in LDS interleaved, so one
+// 16-byte LDS read serves all G rows, and every ELL entry read from the L2-resident table
+// serves them too.  LDS: [(chunk + 1 zero slot) G-vectors] [L*L slice G-vectors].
 template <class T>
 struct AbRows {
   static constexpr int G = 16 / (int)sizeof(T);
@@ -309,28 +318,12 @@ struct AbRows {
   }
 };
 
+// the (ai|..) slice of one row into LDS: element x of row g lives at V[G*x + g]
 template <class T>
-__device__ inline void ab_slice(bool self, uint64_t A, int ai, int L, int64_t LL, const T* __restrict__ eri,
-                                T* __restrict__ V, int g) {
-  // V is the G-vector array; element x of row g lives at V[G*x + g]
+__device__ inline void ab_slice(int ai, int64_t LL, const T* __restrict__ eri, T* __restrict__ V, int g) {
   constexpr int G = AbRows<T>::G;
-  if (self) {
-    for (int x = threadIdx.x; x < LL; x += blockDim.x) {
-      T v = T(0);
-      if (x / L != x % L) {  // the diagonal (kk|jj) part is in hii already
-        uint64_t y = A;
-        while (y) {
-          const int k = __builtin_ctzll(y);
-          y &= y - 1;
-          v += eri[((int64_t)k * L + k) * LL + x];
-        }
-      }
-      V[G * x + g] = v;
-    }
-  } else {
-    const T* src = eri + (int64_t)ai * LL;
-    for (int x = threadIdx.x; x < LL; x += blockDim.x) V[G * x + g] = src[x];
-  }
+  const T* src = eri + (int64_t)ai * LL;
+  for (int x = threadIdx.x; x < LL; x += blockDim.x) V[G * x + g] = src[x];
 }
 
 // Entries of one lane in one (group, chunk): s[g] += sign_e * V[pair_e][g] * cb[jb_e][g] for
@@ -375,17 +368,16 @@ __device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const 
   }
 }
 
-// Alpha-beta and cross-spin single terms of one bra row ia over one column tile:
-//   sigma[ia, ib] += sum_{ja in S(ia)+{ia}} coef * sum_{entries (jb, b<-j) of ib} sb V_ja[b*L+j] c[ja, jb]
-// with V_ja the (ai|..) slice of the alpha single ja (coef its sign) or, for ja = ia, the
-// alpha-occupation slice sum_{k in occ(ia)} (kk|..) (off-diagonal part).  Chunk by chunk of
+// Alpha-beta terms of one bra row ia over one column tile:
+//   sigma[ia, ib] += sum_{ja in S(ia)} sa * sum_{entries (jb, b<-j) of ib} sb (ai|bj) c[ja, jb]
+// (the other-spin occupation parts of the singles are in the row products).  Chunk by chunk of
 // ket columns: G ket rows of the chunk and their slices sit in LDS as interleaved
 // G-vectors, each lane walks its ELL entries for that chunk (the lane -> column map is per
 // chunk, so lanes of a group carry lists of similar length), and the chunk's sums are
 // added into sigma before the next chunk (one owner per element: a column belongs to one
 // tile).
 template <class T, int M>
-__global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, const uint64_t* __restrict__ astr,
+__global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
                                              const int64_t* __restrict__ rpSa,
                                              const int32_t* __restrict__ colSa,
                                              const uint32_t* __restrict__ excSa,
@@ -411,15 +403,14 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
   if (threadIdx.x == 0)
     for (int g = 0; g < G; ++g) cbv[chunk].v[g] = T(0);
   const int64_t q0 = rpSa[ia], q1 = rpSa[ia + 1];
-  const uint64_t A = astr[ia];
   for (int k = 0; k < nch; ++k) {
     const int64_t j0 = (int64_t)k * chunk;
     const int64_t nj = min(nb, j0 + chunk) - j0;
     T acc[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) acc[m] = T(0);
-    for (int64_t t = q0 - 1; t < q1; t += G) {
-      // rows t..t+G-1: t = q0-1 is ia itself; rows at or past q1 are absent (zero)
+    for (int64_t t = q0; t < q1; t += G) {
+      // rows t..t+G-1; rows at or past q1 are absent (zero)
       T coef[G];
       const T* crow[G];
       bool has[G];
@@ -430,14 +421,11 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
         has[g] = q < q1;
         coef[g] = T(0);
         crow[g] = c + ia * nb + j0;
-        if (q < q0) {
-          coef[g] = T(1);
-          ab_slice<T>(true, A, 0, L, LL, eri, Vs, g);
-        } else if (has[g]) {
+        if (has[g]) {
           const uint32_t e = excSa[q];
           coef[g] = ((e >> 24) & 1u) ? T(-1) : T(1);
           crow[g] = c + (int64_t)colSa[q] * nb + j0;
-          ab_slice<T>(false, A, (int)((e >> 6) & 63u) * L + (int)(e & 63u), L, LL, eri, Vs, g);  // a*L + i
+          ab_slice<T>((int)((e >> 6) & 63u) * L + (int)(e & 63u), LL, eri, Vs, g);  // a*L + i
         } else {
           for (int x = threadIdx.x; x < LL; x += NT) Vs[G * x + g] = T(0);
         }
@@ -490,9 +478,9 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles, 
 }
 
 template <class T>
-void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, const T* X, const T* hii,
-                 T* out, hipStream_t st, int64_t cb = 0, int64_t ce = -1, const CoreTable* ct = nullptr,
-                 const T* Z = nullptr, int64_t zcols = 0) {
+void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, const T* J, const T* X,
+                 const T* hii, T* out, hipStream_t st, int64_t cb = 0, int64_t ce = -1,
+                 const CoreTable* ct = nullptr, const T* Z = nullptr, int64_t zcols = 0) {
   if (ce < 0) ce = ncols;
   if (nrows == 0 || ce <= cb) return;
   constexpr int VN = V16<T>::n;
@@ -507,8 +495,8 @@ void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, c
   const int32_t* zs = ct ? ct->slot.p : nullptr;
   dim3 g((unsigned)nblk), b(TS);
 #define SBDG_SPMM(D, V, ZG)                                                                                   \
-  hipLaunchKernelGGL((k_spmm<T, D, V, ZG>), g, b, 0, st, nrows, ncols, cb, ce, tb.rowptr.p, tb.col.p, val, X, \
-                     hii, out, zp, zs, Z, zcols)
+  hipLaunchKernelGGL((k_spmm<T, D, V, ZG>), g, b, 0, st, nrows, ncols, cb, ce, tb.rowptr.p, tb.col.p, val, \
+                     tb.jai.p, J, X, hii, out, zp, zs, Z, zcols)
   if (ct) {
     if (diag) {
       if (vec) SBDG_SPMM(true, true, true);
@@ -543,10 +531,11 @@ int64_t core_zcols(const SigmaPlan& P, const CoreTable& ct, int64_t ncols) {
 // products of one spin (rows of X are that spin's strings): singles + diagonal by the
 // excitation table, doubles by core, one pass per column range of Z
 template <class T>
-void launch_spin(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, const CoreTable* ct, const T* W,
-                 int64_t npair, const T* X, const T* hii, T* out, T* Z, const SigmaPlan& P, hipStream_t st) {
+void launch_spin(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, const T* J, const CoreTable* ct,
+                 const T* W, int64_t npair, const T* X, const T* hii, T* out, T* Z, const SigmaPlan& P,
+                 hipStream_t st) {
   if (!ct) {
-    launch_spmm<T>(diag, nrows, ncols, tb, X, hii, out, st);
+    launch_spmm<T>(diag, nrows, ncols, tb, J, X, hii, out, st);
     return;
   }
   if (nrows == 0 || ncols == 0) return;
@@ -568,7 +557,7 @@ void launch_spin(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, c
         hipLaunchKernelGGL((k_core_prod<T, false>), g, b, 0, st, ncols, cb, ce, ct->start.p, ct->row.p,
                            ct->pair.p, ct->blk.p, ct->blk_x0.p, ct->grp.p, ct->ngrp, W, npair, X, Z, zcols);
     }
-    launch_spmm<T>(diag, nrows, ncols, tb, X, hii, out, st, cb, ce, ct, Z, zcols);
+    launch_spmm<T>(diag, nrows, ncols, tb, J, X, hii, out, st, cb, ce, ct, Z, zcols);
   }
 }
 
@@ -584,7 +573,7 @@ void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* si
   const EllTable& E = *P.ell;
   const int64_t nblk = E.ntiles * P.na;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
-  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.nb, P.L, E.ntiles, P.a,
+  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.nb, P.L, E.ntiles,
                      P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.off.p, E.K.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
                      E.groups, eri, c, sigma);
 }
@@ -641,9 +630,17 @@ void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream
   if constexpr (sizeof(T) == 8) W = reinterpret_cast<const T*>(P.W);
   else W = reinterpret_cast<const T*>(P.W32);
   T* Z = reinterpret_cast<T*>(P.Z);
-  launch_spin<T>(false, P.nb, P.na, *P.mb, P.cb, W, P.npair, cT, nullptr, sT, Z, P, st);
+  const T *Ja, *Jb;
+  if constexpr (sizeof(T) == 8) {
+    Ja = reinterpret_cast<const T*>(P.Ja);
+    Jb = reinterpret_cast<const T*>(P.Jb);
+  } else {
+    Ja = reinterpret_cast<const T*>(P.Ja32);
+    Jb = reinterpret_cast<const T*>(P.Jb32);
+  }
+  launch_spin<T>(false, P.nb, P.na, *P.mb, Ja, P.cb, W, P.npair, cT, nullptr, sT, Z, P, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[2], st));
-  launch_spin<T>(true, P.na, P.nb, *P.ma, P.ca, W, P.npair, c, hii, sigma, Z, P, st);
+  launch_spin<T>(true, P.na, P.nb, *P.ma, Jb, P.ca, W, P.npair, c, hii, sigma, Z, P, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[3], st));
   launch_transpose<T, true>(P.nb, P.na, sT, sigma, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[4], st));

@@ -190,12 +190,23 @@ __global__ void __launch_bounds__(TB) k_double(const uint64_t* __restrict__ s, i
   }
 }
 
-__global__ void k_merge(int64_t n, const int64_t* __restrict__ rps, const int32_t* __restrict__ cs,
+__device__ inline int32_t single_jai(uint32_t e, int L) {
+  const int32_t p = (int32_t)((e >> 6) & 63u) * L + (int32_t)(e & 63u);  // bra-only * L + ket-only
+  return ((e >> 24) & 1u) ? -(p + 1) : (p + 1);
+}
+
+__global__ void k_jai(int64_t nnz, int L, const uint32_t* __restrict__ exc, int32_t* __restrict__ jai) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nnz) jai[q] = single_jai(exc[q], L);
+}
+
+__global__ void k_merge(int64_t n, int L, const int64_t* __restrict__ rps, const int32_t* __restrict__ cs,
                         const double* __restrict__ vs, const float* __restrict__ vs32,
+                        const uint32_t* __restrict__ es,
                         const int64_t* __restrict__ rpd, const int32_t* __restrict__ cd,
                         const double* __restrict__ vd, const float* __restrict__ vd32,
                         int64_t* __restrict__ rpm, int32_t* __restrict__ cm,
-                        double* __restrict__ vm, float* __restrict__ vm32) {
+                        double* __restrict__ vm, float* __restrict__ vm32, int32_t* __restrict__ jm) {
   const int64_t r = blockIdx.x;
   const int64_t b = rps[r] + rpd[r];
   const int64_t ns = rps[r + 1] - rps[r], nd = rpd[r + 1] - rpd[r];
@@ -208,10 +219,12 @@ __global__ void k_merge(int64_t n, const int64_t* __restrict__ rps, const int32_
       cm[b + k] = cs[rps[r] + k];
       vm[b + k] = vs[rps[r] + k];
       vm32[b + k] = vs32[rps[r] + k];
+      jm[b + k] = single_jai(es[rps[r] + k], L);
     } else {
       cm[b + k] = cd[rpd[r] + k - ns];
       vm[b + k] = vd[rpd[r] + k - ns];
       vm32[b + k] = vd32[rpd[r] + k - ns];
+      jm[b + k] = 0;
     }
   }
 }
@@ -245,6 +258,27 @@ __global__ void k_abchunk(int64_t n, int nch, int64_t chunk, const int64_t* __re
 
 // cnt[ib*nch+k]: entries of beta row ib with ket in chunk k, plus the occupied orbitals of
 // ib when ib itself lies in chunk k (pseudo-entries)
+// J[(p+1)*n + r] = sum_{o in occ(s[r])} eri[p*L^2 + o*L + o]; row 0 zero
+__global__ void k_occ_table(const uint64_t* __restrict__ s, int64_t n, int L, const double* __restrict__ eri,
+                            double* __restrict__ J, float* __restrict__ J32) {
+  const int64_t LL = (int64_t)L * L;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (LL + 1) * n) return;
+  const int64_t p = t / n, r = t - p * n;
+  double v = 0.0;
+  if (p > 0) {
+    const double* e = eri + (p - 1) * LL;
+    uint64_t y = s[r];
+    while (y) {
+      const int o = __builtin_ctzll(y);
+      y &= y - 1;
+      v += e[(int64_t)o * L + o];
+    }
+  }
+  J[t] = v;
+  J32[t] = (float)v;
+}
+
 // memberships: thread t -> (string r, electron pair q); key = core bits, value =
 // r | P(a,b) << 32 | neg << 63 with eps = (-1)^(#core below a + #core below b)
 __global__ void k_core_enum(const uint64_t* __restrict__ s, int64_t n, int nel, uint64_t* __restrict__ key,
@@ -389,7 +423,7 @@ void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, co
   SBDG_CHECK(hipStreamSynchronize(st));
 }
 
-void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st, bool with_doubles) {
+void build_spmm(const Table& s, const Table& d, SpmmTable& m, int L, hipStream_t st, bool with_doubles) {
   if (!with_doubles) {  // singles only: copy the singles CSR
     m.n = s.n;
     m.nnz = s.nnz;
@@ -397,11 +431,14 @@ void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st, bo
     m.col.alloc(std::max<int64_t>(m.nnz, 1));
     m.val.alloc(std::max<int64_t>(m.nnz, 1));
     m.val32.alloc(std::max<int64_t>(m.nnz, 1));
+    m.jai.alloc(std::max<int64_t>(m.nnz, 1));
     SBDG_CHECK(hipMemcpyAsync(m.rowptr.p, s.rowptr.p, sizeof(int64_t) * (m.n + 1), hipMemcpyDeviceToDevice, st));
     if (m.nnz) {
       SBDG_CHECK(hipMemcpyAsync(m.col.p, s.col.p, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToDevice, st));
       SBDG_CHECK(hipMemcpyAsync(m.val.p, s.val.p, sizeof(double) * m.nnz, hipMemcpyDeviceToDevice, st));
       SBDG_CHECK(hipMemcpyAsync(m.val32.p, s.val32.p, sizeof(float) * m.nnz, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_jai, dim3(blocks_for(m.nnz, 256)), dim3(256), 0, st, m.nnz, L, s.exc.p, m.jai.p);
+      SBDG_CHECK(hipGetLastError());
     }
     SBDG_CHECK(hipStreamSynchronize(st));
     return;
@@ -412,10 +449,11 @@ void build_spmm(const Table& s, const Table& d, SpmmTable& m, hipStream_t st, bo
   m.col.alloc(std::max<int64_t>(m.nnz, 1));
   m.val.alloc(std::max<int64_t>(m.nnz, 1));
   m.val32.alloc(std::max<int64_t>(m.nnz, 1));
+  m.jai.alloc(std::max<int64_t>(m.nnz, 1));
   if (m.n > 0) {
-    hipLaunchKernelGGL(k_merge, dim3((unsigned)m.n), dim3(TB), 0, st, m.n, s.rowptr.p, s.col.p, s.val.p,
-                       s.val32.p, d.rowptr.p, d.col.p, d.val.p, d.val32.p, m.rowptr.p, m.col.p, m.val.p,
-                       m.val32.p);
+    hipLaunchKernelGGL(k_merge, dim3((unsigned)m.n), dim3(TB), 0, st, m.n, L, s.rowptr.p, s.col.p, s.val.p,
+                       s.val32.p, s.exc.p, d.rowptr.p, d.col.p, d.val.p, d.val32.p, m.rowptr.p, m.col.p, m.val.p,
+                       m.val32.p, m.jai.p);
     SBDG_CHECK(hipGetLastError());
   } else {
     SBDG_CHECK(hipMemsetAsync(m.rowptr.p, 0, sizeof(int64_t), st));
@@ -518,7 +556,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
   }
 }
 
-void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st) {
+void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st) {
   const int64_t nb = sb.n;
   if (chunk >= (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta chunk beyond 2^18 columns");
   if (L > 64) throw Error(SBDG_ERR_LIMIT, "norb beyond 64");
@@ -538,20 +576,14 @@ void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int e
   const int64_t nnz = rp[nb];
   std::vector<int32_t> col(nnz);
   std::vector<uint32_t> exc(nnz);
-  std::vector<uint64_t> bs(nb);
   if (nnz) {
     SBDG_CHECK(hipMemcpy(col.data(), sb.col.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
     SBDG_CHECK(hipMemcpy(exc.data(), sb.exc.p, sizeof(uint32_t) * nnz, hipMemcpyDeviceToHost));
   }
-  if (nb) SBDG_CHECK(hipMemcpy(bs.data(), d_b, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost));
-  // entries of column ib in chunk k: its singles with ket in the chunk, plus, in the chunk
-  // holding ib itself, one pseudo-entry per occupied orbital (alpha-occupation part of the
-  // cross-spin single, pair o*L+o)
+  // entries of column ib in chunk k: its singles with ket in the chunk
   std::vector<int32_t> cnt((size_t)nb * nch, 0);
-  for (int64_t ib = 0; ib < nb; ++ib) {
+  for (int64_t ib = 0; ib < nb; ++ib)
     for (int64_t q = rp[ib]; q < rp[ib + 1]; ++q) ++cnt[(size_t)ib * nch + col[q] / chunk];
-    cnt[(size_t)ib * nch + ib / chunk] += __builtin_popcountll(bs[ib]);
-  }
   // column tiles: columns by decreasing total count, cut into tiles of tw lane positions;
   // inside a tile, each chunk orders its columns by decreasing count in that chunk, so the
   // 64 lanes of a group carry similar lists (little padding).  A column stays in its tile
@@ -589,9 +621,10 @@ void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int e
   const int64_t total = off[ng];
   const uint32_t pad = (uint32_t)chunk;  // zero slot, pair 0
   std::vector<uint32_t> ent((size_t)std::max<int64_t>(total, 1), pad);
-  // LDS pass groups of the kernel's row-vector reads: 16-byte vectors in groups of 8 lanes
-  // over 8 slots, 8-byte vectors in groups of 16 lanes over 16 slots
-  const int P = elem >= 16 ? 8 : 16, S = P;
+  // pass groups of the kernel's 16-byte LDS reads, as measured on gfx950 (the alpha-beta
+  // kernel at norb=36, 20,000^2 with four candidate models; this one was fastest): 32 lanes
+  // over 16 slots of 16 bytes (256 bytes of banks)
+  const int P = elem >= 16 ? 32 : 16, S = 16;
   const uint32_t vbase = (uint32_t)(chunk + 1);  // integral pairs follow the ket pairs
   std::vector<std::vector<uint32_t>> lists(64);
   for (int64_t g = 0; g < e.groups; ++g)
@@ -608,14 +641,6 @@ void build_ell(const Table& sb, const uint64_t* d_b, int L, int64_t chunk, int e
           const uint32_t x = exc[q];
           const uint32_t j = x & 63u, b = (x >> 6) & 63u, neg = (x >> 24) & 1u;
           lst.push_back((uint32_t)(col[q] - j0) | ((b * (uint32_t)L + j) << 18) | (neg << 30));
-        }
-        if (ib / chunk == k) {
-          uint64_t y = bs[ib];
-          while (y) {
-            const uint32_t o = (uint32_t)__builtin_ctzll(y);
-            y &= y - 1;
-            lst.push_back((uint32_t)(ib - j0) | ((o * (uint32_t)L + o) << 18));
-          }
         }
       }
       schedule_block(lists, K[g * nch + k], P, S, vbase, pad, ent.data() + base);
@@ -729,6 +754,14 @@ void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct
   up(ct.slot_ptr, slot_ptr);
   up(ct.slot, slot);
   up(ct.grp, grp);
+}
+
+void build_occ_table(const uint64_t* d_str, int64_t n, int L, const double* d_eri, double* J, float* J32,
+                     hipStream_t st) {
+  const int64_t total = ((int64_t)L * L + 1) * n;
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_occ_table, dim3(blocks_for(total, 256)), dim3(256), 0, st, d_str, n, L, d_eri, J, J32);
+  SBDG_CHECK(hipGetLastError());
 }
 
 void build_pair_integrals(const double* d_eri, int L, double* W, float* W32, hipStream_t st) {

@@ -180,6 +180,24 @@ def test_sigma_c1_shape_row_sample(doubles, core_gib):
         assert np.abs(sx32[blk] - sx[blk]).max() <= 2e-5 * np.abs(sx[blk]).max()
 
 
+def test_sigma_wide_beta_set_vs_oracle():
+    """30,000 beta strings (norb=24, 5 electrons) against 12 alpha strings: the alpha-beta
+    kernel then spans several column tiles (24,576 columns each), each with its own
+    per-chunk lane order."""
+    rng = np.random.default_rng(77)
+    L = 24
+    h1, eri = syn.synthetic_integrals(L, rng)
+    astr = syn.sample_strings(L, 5, 12, rng)
+    bstr = syn.sample_strings(L, 5, 30000, rng)
+    case = dict(norb=L, core=0.5, h1=h1, eri=eri, astr=astr, bstr=bstr)
+    s, p = solver(case), oproblem(case)
+    x = rng.normal(size=s.ndet)
+    sx, ox = s.sigma(x), p.sigma(x)
+    assert np.abs(sx - ox).max() <= 1e-12 * np.abs(ox).max()
+    sx32 = s.sigma(x, precision=1)
+    assert np.abs(sx32 - ox).max() <= 2e-5 * np.abs(ox).max()
+
+
 def test_core_doubles_energy_equals_table_doubles():
     """Davidson energies with either doubles formulation agree to round-off."""
     case = medium_case()
