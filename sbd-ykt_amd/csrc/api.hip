@@ -282,13 +282,21 @@ void initial_vector(sbdg_ctx* c, double* W) {
 double expand(sbdg_ctx* c, Small& s, int m, int64_t N, const double* theta_dev, const double* rn2_dev) {
   double* vm = c->V.p + (int64_t)m * N;
   precond_impl(c->R.p, rn2_dev, c->d_hii.p, theta_dev, vm, N, c->st);
-  double nt = 0.0;
-  for (int pass = 0; pass < 2; ++pass) {
-    multi_dot_impl(c->V.p, N, m, vm, N, s.dots, s.part, c->st);
-    gs_impl(c->V.p, N, m, s.dots, vm, N, s.norms + 8, s.part, c->st);
-  }
+  // classical Gram-Schmidt against V[0..m); the new vector t sits at V[m], so one more
+  // column of the same dot pass gives |t|^2 before projection.  A second pass runs only
+  // when the first removed most of t (|t1| < |t0| / sqrt(2)): "twice is enough".
+  double nt = 0.0, n0 = 0.0;
+  multi_dot_impl(c->V.p, N, m + 1, vm, N, s.dots, s.part, c->st);
+  gs_impl(c->V.p, N, m, s.dots, vm, N, s.norms + 8, s.part, c->st);
+  SBDG_CHECK(hipMemcpyAsync(&n0, s.dots + m, sizeof(double), hipMemcpyDeviceToHost, c->st));
   SBDG_CHECK(hipMemcpyAsync(&nt, s.norms + 8, sizeof(double), hipMemcpyDeviceToHost, c->st));
   SBDG_CHECK(hipStreamSynchronize(c->st));
+  if (!(nt >= 0.5 * n0)) {
+    multi_dot_impl(c->V.p, N, m, vm, N, s.dots, s.part, c->st);
+    gs_impl(c->V.p, N, m, s.dots, vm, N, s.norms + 8, s.part, c->st);
+    SBDG_CHECK(hipMemcpyAsync(&nt, s.norms + 8, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+  }
   if (!std::isfinite(nt)) throw Error(SBDG_ERR_NUMERIC, "non-finite correction vector");
   if (nt <= 0.0) return 0.0;
   scale_impl(vm, s.norms + 8, N, c->st);

@@ -348,13 +348,17 @@ __device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const 
 #pragma unroll
         for (int u = 0; u < 4; ++u) pn[u] = ep[(uint32_t)(e + 4 + u) * 64u];
       }
+      RV v[4], x[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const RV v = Vv[(p[u] >> 18) & 0xFFFu];
-        const RV x = cbv[p[u] & 0x3FFFFu];
+        v[u] = Vv[(p[u] >> 18) & 0xFFFu];
+        x[u] = cbv[p[u] & 0x3FFFFu];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
         const uint32_t neg = (p[u] >> 30) & 1u;
 #pragma unroll
-        for (int g = 0; g < G; ++g) s[g] += R::flip(v.v[g], neg) * x.v[g];
+        for (int g = 0; g < G; ++g) s[g] += R::flip(v[u].v[g], neg) * x[u].v[g];
       }
     }
   }
