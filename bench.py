@@ -46,14 +46,15 @@ def kernel_bytes(na, nb, nnz, core, nel_b, s):
     """Algorithmic HBM bytes per launch of each sigma phase (s = bytes per element).
 
     transpose:     read c, write c^T                                   2 N s
-    beta product:  on c^T: singles rows + diagonal-free write          s na nnzSb + N s + 12 nnzSb
+    beta product:  on c^T: singles rows and their occupation-table
+                   rows, diagonal-free write                           2 s na nnzSb + N s + 16 nnzSb
                    doubles by core (mb memberships): member rows in,
                    partial rows out, partial rows gathered              + 3 s na mb
                    (excitation-table form instead: s na nnzDb + 12 nnzDb)
-    alpha product: same on c, with hii and c read for the diagonal     s nb nnzSa + 3 N s + ...
+    alpha product: same on c, with hii and c read for the diagonal     2 s nb nnzSa + 3 N s + ...
     transpose-add: read sigma^T and sigma, write sigma                 3 N s
-    alpha-beta:    one ket row per (ia, ja) pair incl. ia, sigma
-                   read and written, ELL entries                       s nb (na + nnzSa) + 2 N s + 4 (nnzSb + nb nel_b)
+    alpha-beta:    one ket row per alpha single (ia, ja), sigma read
+                   and written, ELL entries                            s nb nnzSa + 2 N s + 4 nnzSb
     """
     N = na * nb
     sa, da, sb, db = nnz
@@ -61,24 +62,23 @@ def kernel_bytes(na, nb, nnz, core, nel_b, s):
 
     def spin(ncols, singles, doubles, members):
         if members:
-            return s * ncols * (singles + 3 * members) + 12 * singles
-        return s * ncols * (singles + doubles) + 12 * (singles + doubles)
+            return s * ncols * (2 * singles + 3 * members) + 16 * singles
+        return s * ncols * (2 * singles + doubles) + 16 * (singles + doubles)
 
     return {
         "transpose": 2 * N * s,
         "beta": spin(na, sb, db, mb) + N * s,
         "alpha": spin(nb, sa, da, ma) + 3 * N * s,
         "transpose_add": 3 * N * s,
-        "alpha_beta": s * nb * (na + sa) + 2 * N * s + 4 * (sb + nb * nel_b),
+        "alpha_beta": s * nb * sa + 2 * N * s + 4 * sb,
     }
 
 
 def ab_lds_bytes(na, nb, nnz, nel_b, s):
-    """LDS bytes the alpha-beta kernel reads per launch: every (bra row, ket row) pair of
-    the alpha singles plus the row itself, times every ELL entry (beta singles plus one
-    occupation entry per beta electron), one ket value and one integral each."""
+    """LDS bytes the alpha-beta kernel reads per launch: every alpha single (bra row, ket
+    row) times every beta single of the bra column, one ket value and one integral each."""
     sa, _, sb, _ = nnz
-    return (sa + na) * (sb + nb * nel_b) * 2 * s
+    return sa * sb * 2 * s
 
 
 def survey_b_alg(na, nb, nnz, s):

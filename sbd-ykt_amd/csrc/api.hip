@@ -279,16 +279,15 @@ void initial_vector(sbdg_ctx* c, double* W) {
 
 // t = precondition(R) then classical Gram-Schmidt twice against V[0..m), normalised
 // into V[m].  Returns |t| before the final normalisation.
-double expand(sbdg_ctx* c, Small& s, int m, int64_t N, const double* theta_dev, const double* rn2_dev) {
+// Gram-Schmidt of t = V[m] against V[0..m) given the first-pass dots in s.dots[0..m) and
+// |t|^2 before projection at n0_dev; normalises t and returns |t| after projection (0 when
+// nothing is left).  A second pass runs only when the first removed most of t
+// (|t1| < |t0| / sqrt(2)): "twice is enough".
+double finish_gs(sbdg_ctx* c, Small& s, int m, int64_t N, const double* n0_dev) {
   double* vm = c->V.p + (int64_t)m * N;
-  precond_impl(c->R.p, rn2_dev, c->d_hii.p, theta_dev, vm, N, c->st);
-  // classical Gram-Schmidt against V[0..m); the new vector t sits at V[m], so one more
-  // column of the same dot pass gives |t|^2 before projection.  A second pass runs only
-  // when the first removed most of t (|t1| < |t0| / sqrt(2)): "twice is enough".
   double nt = 0.0, n0 = 0.0;
-  multi_dot_impl(c->V.p, N, m + 1, vm, N, s.dots, s.part, c->st);
   gs_impl(c->V.p, N, m, s.dots, vm, N, s.norms + 8, s.part, c->st);
-  SBDG_CHECK(hipMemcpyAsync(&n0, s.dots + m, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  SBDG_CHECK(hipMemcpyAsync(&n0, n0_dev, sizeof(double), hipMemcpyDeviceToHost, c->st));
   SBDG_CHECK(hipMemcpyAsync(&nt, s.norms + 8, sizeof(double), hipMemcpyDeviceToHost, c->st));
   SBDG_CHECK(hipStreamSynchronize(c->st));
   if (!(nt >= 0.5 * n0)) {
@@ -301,6 +300,15 @@ double expand(sbdg_ctx* c, Small& s, int m, int64_t N, const double* theta_dev, 
   if (nt <= 0.0) return 0.0;
   scale_impl(vm, s.norms + 8, N, c->st);
   return std::sqrt(nt);
+}
+
+// correction vector from the stored residual R: preconditioned, then Gram-Schmidt (the new
+// vector sits at V[m], so one more column of the dot pass gives |t|^2 before projection)
+double expand(sbdg_ctx* c, Small& s, int m, int64_t N, const double* theta_dev, const double* rn2_dev) {
+  double* vm = c->V.p + (int64_t)m * N;
+  precond_impl(c->R.p, rn2_dev, c->d_hii.p, theta_dev, vm, N, c->st);
+  multi_dot_impl(c->V.p, N, m + 1, vm, N, s.dots, s.part, c->st);
+  return finish_gs(c, s, m, N, s.dots + m);
 }
 
 void davidson(sbdg_ctx* c) {
@@ -379,6 +387,7 @@ void davidson(sbdg_ctx* c) {
     return el;
   };
   // projected-matrix eigenpair and Ritz vector/residual over the first m vectors
+  bool w_valid = false;  // W holds the Ritz vector of the current projected matrix
   auto ritz = [&](int m) {
     SBDG_CHECK(hipMemcpyAsync(s.H, Hs.data(), sizeof(double) * MAXSUB * MAXSUB, hipMemcpyHostToDevice, c->st));
     syev_device(s.H, m, MAXSUB, s.E, s.U, c->st);
@@ -397,6 +406,35 @@ void davidson(sbdg_ctx* c) {
     SBDG_CHECK(hipMemcpyAsync(rr_exp, s.norms + 1, sizeof(double), hipMemcpyDeviceToDevice, c->st));
     theta = E[0];
     rnorm = std::sqrt(nr[1]);
+    w_valid = true;
+  };
+  // a step that goes on to expand without restart: Ritz pair, residual norm and the
+  // preconditioned correction with its first Gram-Schmidt dots in one pass (t -> V[m]);
+  // the Ritz vector W is formed later only if needed (w_valid)
+  auto ritz_fused = [&](int m) {
+    SBDG_CHECK(hipMemcpyAsync(s.H, Hs.data(), sizeof(double) * MAXSUB * MAXSUB, hipMemcpyHostToDevice, c->st));
+    syev_device(s.H, m, MAXSUB, s.E, s.U, c->st);
+    double* tm = c->V.p + (int64_t)m * N;
+    if (mixed)
+      ritz_expand_impl<float>(c->V.p, N, c->HV32.p, N, m, s.U, s.E, c->d_hii.p, tm, N, s.dots, s.part, c->st);
+    else
+      ritz_expand_impl<double>(c->V.p, N, c->HV.p, N, m, s.U, s.E, c->d_hii.p, tm, N, s.dots, s.part, c->st);
+    double rr = 0.0;
+    SBDG_CHECK(hipMemcpyAsync(&rr, s.dots + m, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipMemcpyAsync(E.data(), s.E, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    if (!std::isfinite(rr) || !std::isfinite(E[0])) throw Error(SBDG_ERR_NUMERIC, "non-finite Ritz value or residual");
+    theta = E[0];
+    rnorm = std::sqrt(rr);
+    w_valid = false;
+  };
+  auto finish_fused = [&]() {
+    const double nt = finish_gs(c, s, v, N, s.dots + v + 1);
+    if (nt == 0.0) {
+      stop = true;  // the residual lies in the span of the basis
+      return;
+    }
+    ++v;
   };
   // mixed: FP64 sigma of basis vector 0 (= current Ritz vector): exact row/column 0 of
   // the projected matrix, exact Rayleigh quotient and residual
@@ -466,7 +504,10 @@ void davidson(sbdg_ctx* c) {
     for (int k = 0; k <= h; ++k) Hs[k + MAXSUB * h] = Hs[h + MAXSUB * k] = dots[k];
     ++h;
     ++since;
-    ritz(h);
+    const bool do_refresh = mixed && (since >= std::max(1, o.refresh_k) || v == nbmax);
+    const bool fuse = v < nbmax && !do_refresh && h <= 32;  // expansion follows directly
+    if (fuse) ritz_fused(h);
+    else ritz(h);
     if (!mixed) {
       const double el = log_step(true);
       if (rnorm < o.eps) {
@@ -484,13 +525,14 @@ void davidson(sbdg_ctx* c) {
         for (int k = 0; k < keep; ++k) Hs[k + MAXSUB * k] = E[k];
         v = h = keep;
       }
-      expand_or_stop();
+      if (fuse) finish_fused();
+      else expand_or_stop();
     } else {
-      const bool do_refresh = since >= std::max(1, o.refresh_k) || v == nbmax;
       if (!do_refresh) {
         const double el = log_step(false);
         if (check_time(el, ts)) break;
-        expand_or_stop();
+        if (fuse) finish_fused();
+        else expand_or_stop();
         continue;
       }
       // the basis becomes the Ritz vectors of the subspace (all of them, or `keep` when
@@ -519,6 +561,7 @@ void davidson(sbdg_ctx* c) {
       expand_or_stop();
     }
   }
+  if (!w_valid) ritz(h);  // the last step was fused: form the Ritz vector
   c->stats.converged = converged ? 1 : 0;
   c->stats.t_davidson = now() - tstart;
   SBDG_CHECK(hipEventDestroy(e0));

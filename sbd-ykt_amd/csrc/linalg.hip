@@ -145,6 +145,62 @@ __global__ void __launch_bounds__(RT) k_ritz(const double* __restrict__ V, int64
   }
 }
 
+// One pass over the basis for a step that continues without restart: with the Ritz pair
+// (theta, u) of the projected matrix, r = sum_k u_k HV_k - theta sum_k u_k V_k,
+// t = r / (theta - hii) (regularised as k_precond; the missing 1/|r| is a common factor
+// that the normalisation after Gram-Schmidt removes) written to T, and the partials of
+// V_k . t (k < m), |r|^2 and |t|^2.  The Ritz vector itself is not formed.
+template <class TH, int MM>
+__global__ void __launch_bounds__(RT) k_ritz_expand(const double* __restrict__ V, int64_t ldv,
+                                                    const TH* __restrict__ HV, int64_t ldh, int m,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ theta,
+                                                    const double* __restrict__ hii, double* __restrict__ T,
+                                                    int64_t n, double* __restrict__ part) {
+  __shared__ double sh[RT / 64];
+  double uk[MM], d[MM];
+#pragma unroll
+  for (int k = 0; k < MM; ++k) {
+    uk[k] = (k < m) ? u[k] : 0.0;
+    d[k] = 0.0;
+  }
+  const double th = theta[0];
+  const double eps_reg = 1.0e-12;
+  double sr = 0.0, st = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+    double w = 0.0, h = 0.0;
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m) {
+        w += uk[k] * V[k * ldv + i];
+        h += uk[k] * (double)HV[k * ldh + i];
+      }
+    const double r = h - th * w;
+    const double den = th - hii[i];
+    const double t = (fabs(den) > eps_reg) ? r / den : r / (den - eps_reg);
+    T[i] = t;
+    sr += r * r;
+    st += t * t;
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m) d[k] += V[k * ldv + i] * t;
+  }
+#pragma unroll
+  for (int k = 0; k < MM; ++k) {
+    if (k < m) {
+      const double v = block_sum(d[k], sh);
+      if (threadIdx.x == 0) part[(int64_t)blockIdx.x * (m + 2) + k] = v;
+    }
+  }
+  sr = block_sum(sr, sh);
+  st = block_sum(st, sh);
+  if (threadIdx.x == 0) {
+    part[(int64_t)blockIdx.x * (m + 2) + m] = sr;
+    part[(int64_t)blockIdx.x * (m + 2) + m + 1] = st;
+  }
+}
+
 __global__ void k_scale(double* __restrict__ x, const double* __restrict__ n2, int64_t n) {
   const double f = 1.0 / sqrt(n2[0]);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
@@ -385,6 +441,27 @@ template void ritz_combine_impl<double>(const double*, int64_t, const double*, i
                                         const double*, double*, double*, int64_t, double*, double*, hipStream_t);
 template void ritz_combine_impl<float>(const double*, int64_t, const float*, int64_t, int, const double*,
                                        const double*, double*, double*, int64_t, double*, double*, hipStream_t);
+
+template <class TH>
+void ritz_expand_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, int m, const double* u,
+                      const double* theta, const double* hii, double* T, int64_t n, double* out, double* part,
+                      hipStream_t st) {
+  if (m > 32) throw Error(SBDG_ERR_LIMIT, "fused Ritz step beyond 32 vectors");
+  const unsigned g = nblocks(n);
+  dispatch_m<0>(m, [&](auto MM) {
+    if constexpr (decltype(MM)::value <= 32)
+      hipLaunchKernelGGL((k_ritz_expand<TH, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u,
+                         theta, hii, T, n, part);
+  });
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, m + 2, out);
+  SBDG_CHECK(hipGetLastError());
+}
+template void ritz_expand_impl<double>(const double*, int64_t, const double*, int64_t, int, const double*,
+                                       const double*, const double*, double*, int64_t, double*, double*,
+                                       hipStream_t);
+template void ritz_expand_impl<float>(const double*, int64_t, const float*, int64_t, int, const double*,
+                                      const double*, const double*, double*, int64_t, double*, double*,
+                                      hipStream_t);
 
 void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st) {
   if (n == 0) return;

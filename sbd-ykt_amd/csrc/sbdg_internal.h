@@ -204,6 +204,11 @@ template <class TH>
 void ritz_combine_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, int m, const double* u,
                        const double* theta, double* W, double* R, int64_t n, double* out, double* part,
                        hipStream_t st);
+// fused Ritz step (m <= 32): out[0..m) = V_k . t, out[m] = |r|^2, out[m+1] = |t|^2, t -> T
+template <class TH>
+void ritz_expand_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, int m, const double* u,
+                      const double* theta, const double* hii, double* T, int64_t n, double* out, double* part,
+                      hipStream_t st);
 void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st);  // x /= sqrt(n2[0])
 // t = (R/|R|) / (theta - d), with the 1e-12 regularisation of davidson.h:855-862
 void precond_impl(const double* R, const double* rn2, const double* hii, const double* theta, double* t, int64_t n,
