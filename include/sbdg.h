@@ -56,7 +56,9 @@ typedef struct {
   uint64_t seed;      /* init==1: random start vector seed */
   int doubles;        /* same-spin doubles in sigma: 0 auto, 1 excitation tables (as
                          mult), 2 grouped by their (n-2)-electron core */
-  double core_gib;    /* doubles==2: workspace cap in GiB for the per-core products */
+  double core_gib;    /* workspace cap in GiB for the per-core products (default 16); with
+                         doubles==0 a spin whose 512-column pass does not fit uses the
+                         excitation-table form, with doubles==2 it is an error */
 } sbdg_options;
 
 typedef struct {

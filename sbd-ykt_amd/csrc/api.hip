@@ -167,8 +167,12 @@ void build(sbdg_ctx* c) {
   if (c->opt.doubles != 1) {
     build_cores(c->d_a.p, na, L, nel_a, c->Ca, c->st);
     build_cores(c->d_b.p, nb, L, nel_b, c->Cb, c->st);
-    c->use_ca = c->opt.doubles == 2 || c->Da.nnz > 4 * c->Ca.nmem;
-    c->use_cb = c->opt.doubles == 2 || c->Db.nnz > 4 * c->Cb.nmem;
+    // auto: the core form where it moves fewer rows and one 512-column pass of its
+    // workspace fits core_gib; otherwise the excitation-table form
+    const size_t cap = (size_t)(c->opt.core_gib * 1073741824.0);
+    auto fits = [&](const CoreTable& ct) { return (size_t)ct.nmem * CORE_COLS * sizeof(double) <= cap; };
+    c->use_ca = c->opt.doubles == 2 || (c->Da.nnz > 4 * c->Ca.nmem && fits(c->Ca));
+    c->use_cb = c->opt.doubles == 2 || (c->Db.nnz > 4 * c->Cb.nmem && fits(c->Cb));
   }
   build_spmm(c->Sa, c->Da, c->Ma, L, c->st, !c->use_ca);
   build_spmm(c->Sb, c->Db, c->Mb, L, c->st, !c->use_cb);
@@ -702,7 +706,7 @@ int sbdg_default_options(sbdg_options* o) {
   o->verbose = 0;
   o->seed = 12345;
   o->doubles = 0;
-  o->core_gib = 8.0;
+  o->core_gib = 16.0;
   return SBDG_OK;
 }
 

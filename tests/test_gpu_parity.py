@@ -198,6 +198,26 @@ def test_sigma_wide_beta_set_vs_oracle():
     assert np.abs(sx32 - ox).max() <= 2e-5 * np.abs(ox).max()
 
 
+def test_core_workspace_cap():
+    """A workspace cap below one 512-column pass: auto mode falls back to the
+    excitation-table form (same sigma), forcing the core form is a LIMIT error."""
+    case = medium_case()
+    p = oproblem(case)
+    x = np.random.default_rng(4).normal(size=len(case["astr"]) * len(case["bstr"]))
+    ox = p.sigma(x)
+    s = solver(case, doubles=0, core_gib=1e-6)
+    s.build()
+    assert s.stats()["core_members"] == [0, 0]
+    assert np.abs(s.sigma(x) - ox).max() <= 1e-12 * np.abs(ox).max()
+    s2 = solver(case, doubles=0)
+    s2.build()
+    assert min(s2.stats()["core_members"]) > 0  # the default cap takes the core form here
+    s3 = solver(case, doubles=2, core_gib=1e-6)
+    with pytest.raises(sbdg.SbdgError) as ei:
+        s3.build()
+    assert ei.value.code == sbdg.ERR_LIMIT
+
+
 def test_core_doubles_energy_equals_table_doubles():
     """Davidson energies with either doubles formulation agree to round-off."""
     case = medium_case()
