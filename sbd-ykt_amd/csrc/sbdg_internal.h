@@ -86,9 +86,9 @@ struct SpmmTable {
   DBuf<int32_t> jai;
 };
 
-// Packed beta singles for the alpha-beta kernel: bits 0-17 ket index jb, 18-29 orbital
-// pair b*64+j, bit 30 sign.  cptr[ib*(nch+1)+k] is the first entry of row ib whose ket
-// lies in column chunk k (entries ascend in jb, so chunks are contiguous runs).
+// Packed beta singles for the alpha-beta RDM kernel: bits 0-17 ket index jb, 18-29
+// orbital pair b*L+j, bit 30 sign.  cptr[ib*(nch+1)+k] is the first entry of row ib whose
+// ket lies in column chunk k (entries ascend in jb, so chunks are contiguous runs).
 struct AbTable {
   int nch = 0;
   int64_t chunk = 0;
@@ -96,16 +96,15 @@ struct AbTable {
   DBuf<int32_t> cptr;
 };
 
-// Beta singles in an ELL layout for the alpha-beta sigma kernel.  For every group g of 64
-// consecutive bra columns and column chunk k there are K[g*nch+k] entries per lane (a
-// multiple of 4), stored lane-interleaved at off[g*nch+k] + e*64 + lane and padded with a
-// zero-weight entry, so a wave reads them coalesced with a wave-uniform trip count.  Entry: bits 0-17 ket column
-// offset inside the chunk (the value `chunk` addresses a zero slot: padding), bits 18-29
-// orbital pair b*L+j, bit 30 sign.  The beta-occupation part of the alpha singles rides
-// along as pseudo-entries: ket column = bra column, pair k*L+k for each occupied k.
-// Lane positions are a permutation of the columns (colperm: position -> column) sorted by
-// decreasing number of beta singles, so a 64-lane group holds columns of similar length
-// and the padding stays small when the connectivity is uneven (sampled string sets).
+// Beta singles in an ELL layout for the alpha-beta sigma kernel.  Columns are cut into
+// tiles of nt*m lane positions (one workgroup each); in ket chunk k, colperm[k*npos + pos]
+// is the column at lane position pos (columns sorted by their entry count in chunk k
+// inside the tile, so a 64-lane group holds lists of similar length; -1: none).  Group g
+// has K[g*nch+k] entries per lane, stored lane-interleaved at off[g*nch+k] + e*64 + lane
+// and padded with a zero-weight entry, so a wave reads them coalesced with a wave-uniform
+// trip count.  Entry: bits 0-17 ket column offset inside the chunk (the value `chunk`
+// addresses a zero slot: padding), bits 18-29 orbital pair b*L+j, bit 30 sign.  Within a
+// lane the entries are ordered so that lanes read distinct LDS slots where possible.
 struct EllTable {
   int nch = 0;
   int nt = 0, m = 0;        // alpha-beta launch: threads per workgroup, columns per thread
@@ -135,7 +134,7 @@ struct CoreTable {
   int64_t ngrp = 0;
   DBuf<int32_t> grp;      // ngrp + 1: product blocks of each workgroup
 };
-constexpr int CORE_XB = 8;      // output members per product block
+constexpr int CORE_XB = 16;     // output members per product block
 constexpr int CORE_COLS = 512;  // column granularity of the core-product passes
 constexpr int CORE_WORK = 256;  // member rows streamed per core-product workgroup (target)
 
@@ -185,7 +184,7 @@ struct SigmaPlan {
 template <class T>
 void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream_t st,
                  hipEvent_t* ev = nullptr);
-size_t ab_lds_bytes(int L, int64_t chunk, int elem);
+size_t ab_lds_bytes(int L, int64_t chunk);
 
 // vector kernels (linalg.hip)
 void sign_transform(const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, const double* in,

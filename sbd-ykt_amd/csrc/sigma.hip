@@ -587,7 +587,7 @@ void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream
   if (P.na == 0 || P.nb == 0) return;
   const EllTable& E = *P.ell;
   if ((int64_t)E.nt * E.m * E.ntiles != E.groups * 64) throw Error(SBDG_ERR_STATE, "alpha-beta table geometry");
-  const size_t lds = ab_lds_bytes(P.L, E.chunk, (int)sizeof(T));
+  const size_t lds = ab_lds_bytes(P.L, E.chunk);
   switch (E.m) {  // columns per thread, fixed when the table was built
     case 1: launch_ab_m<T, 1>(P, lds, eri, c, sigma, st); break;
     case 2: launch_ab_m<T, 2>(P, lds, eri, c, sigma, st); break;
@@ -608,9 +608,8 @@ void transpose_d(int64_t R, int64_t C, const double* in, double* out, hipStream_
   SBDG_CHECK(hipGetLastError());
 }
 
-size_t ab_lds_bytes(int L, int64_t chunk, int elem) {
+size_t ab_lds_bytes(int L, int64_t chunk) {
   return (size_t)16 * ((size_t)chunk + 1 + (size_t)L * L);  // 16-byte row vectors, any precision
-  (void)elem;
 }
 
 template <class T>

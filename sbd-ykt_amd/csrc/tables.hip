@@ -256,8 +256,6 @@ __global__ void k_abchunk(int64_t n, int nch, int64_t chunk, const int64_t* __re
 }
 
 
-// cnt[ib*nch+k]: entries of beta row ib with ket in chunk k, plus the occupied orbitals of
-// ib when ib itself lies in chunk k (pseudo-entries)
 // J[(p+1)*n + r] = sum_{o in occ(s[r])} eri[p*L^2 + o*L + o]; row 0 zero
 __global__ void k_occ_table(const uint64_t* __restrict__ s, int64_t n, int L, const double* __restrict__ eri,
                             double* __restrict__ J, float* __restrict__ J32) {
