@@ -36,7 +36,7 @@ using namespace sbdg;
 
 namespace {
 constexpr int MAXSUB = 48;
-constexpr size_t LDS_BUDGET = 144 * 1024;  // alpha-beta kernel, one 1024-thread workgroup per CU
+constexpr size_t LDS_BUDGET = 160 * 1024;  // alpha-beta kernel: all of a CU's LDS, one 1024-thread workgroup
 }  // namespace
 
 struct sbdg_ctx {

@@ -98,19 +98,24 @@ struct AbTable {
 
 // Beta singles in an ELL layout for the alpha-beta sigma kernel.  Columns are cut into
 // tiles of nt*m lane positions (one workgroup each); in ket chunk k, colperm[k*npos + pos]
-// is the column at lane position pos (columns sorted by their entry count in chunk k
-// inside the tile, so a 64-lane group holds lists of similar length; -1: none).  Group g
-// has K[g*nch+k] entries per lane, stored lane-interleaved at off[g*nch+k] + e*64 + lane
-// and padded with a zero-weight entry, so a wave reads them coalesced with a wave-uniform
-// trip count.  Entry: bits 0-17 ket column offset inside the chunk (the value `chunk`
-// addresses a zero slot: padding), bits 18-29 orbital pair b*L+j, bit 30 sign.  Within a
-// lane the entries are ordered so that lanes read distinct LDS slots where possible.
+// is the column at lane position pos (-1: none).  Per chunk, a tile's columns are sorted by
+// their entry count in that chunk and cut into 64-lane groups (lists of similar length);
+// wave w of tile t walks the groups at slots g = t*W*m + w + j*W, j < m (W = nt/64), and
+// the groups are dealt to the slots longest first, each to the wave with the fewest rows
+// so far, so the waves of a workgroup reach its barriers together.  Per chunk k a wave's
+// groups form one stream starting at soff[(t*W + w)*nch + k], group after group, each a
+// multiple of four rows of 64 lane-interleaved entries (nq[stream*m + j] batches of four
+// rows) padded with a zero-weight entry: the wave reads its stream coalesced, four rows
+// at a time, with the next four rows in flight across group boundaries.
+// Entry: bits 0-17 ket column offset inside the chunk (the value `chunk` addresses a zero
+// slot: padding), bits 18-29 orbital pair b*L+j, bit 30 sign.  Within a lane the entries
+// are ordered so that lanes read distinct LDS slots where possible.
 struct EllTable {
   int nch = 0;
   int nt = 0, m = 0;        // alpha-beta launch: threads per workgroup, columns per thread
   int64_t chunk = 0, groups = 0, ntiles = 0;
-  DBuf<int64_t> off;        // groups*nch + 1
-  DBuf<int32_t> K;          // groups*nch
+  DBuf<int64_t> soff;       // ntiles*(nt/64)*nch + 1: start of each wave's entry stream per chunk
+  DBuf<int32_t> nq;         // ntiles*(nt/64)*nch*m: four-row batches of each group of the stream
   DBuf<uint32_t> ent;
   DBuf<int32_t> colperm;    // nch * groups*64: column of each lane position in each chunk (-1: none)
 };

@@ -326,49 +326,25 @@ __device__ inline void ab_slice(int ai, int64_t LL, const T* __restrict__ eri, T
   for (int x = threadIdx.x; x < LL; x += blockDim.x) V[G * x + g] = src[x];
 }
 
-// Entries of one lane in one (group, chunk): s[g] += sign_e * V[pair_e][g] * cb[jb_e][g] for
-// the G staged rows at once.  n is wave-uniform; four entries are in flight while the
-// previous four are used.
+// One batch of four stream rows of one lane: s[g] += sign_e * V[pair_e][g] * cb[jb_e][g] for
+// the G staged rows at once (all eight LDS reads issued before the products).
 template <class T>
-__device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const typename AbRows<T>::V* cbv,
-                                  const typename AbRows<T>::V* Vv, T (&s)[AbRows<T>::G]) {
+__device__ inline void ab_batch(const uint32_t (&p)[4], const typename AbRows<T>::V* cbv,
+                                const typename AbRows<T>::V* Vv, T (&s)[AbRows<T>::G]) {
   using R = AbRows<T>;
   using RV = typename R::V;
   constexpr int G = R::G;
-  int e = 0;
-  if (n >= 4) {
-    uint32_t pn[4];
+  RV v[4], x[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) pn[u] = ep[(uint32_t)u * 64u];
-    for (; e + 4 <= n; e += 4) {
-      uint32_t p[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) p[u] = pn[u];
-      if (e + 8 <= n) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) pn[u] = ep[(uint32_t)(e + 4 + u) * 64u];
-      }
-      RV v[4], x[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        v[u] = Vv[(p[u] >> 18) & 0xFFFu];
-        x[u] = cbv[p[u] & 0x3FFFFu];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t neg = (p[u] >> 30) & 1u;
-#pragma unroll
-        for (int g = 0; g < G; ++g) s[g] += R::flip(v[u].v[g], neg) * x[u].v[g];
-      }
-    }
+  for (int u = 0; u < 4; ++u) {
+    v[u] = Vv[(p[u] >> 18) & 0xFFFu];
+    x[u] = cbv[p[u] & 0x3FFFFu];
   }
-  for (; e < n; ++e) {
-    const uint32_t p = ep[(uint32_t)e * 64u];
-    const RV v = Vv[(p >> 18) & 0xFFFu];
-    const RV x = cbv[p & 0x3FFFFu];
-    const uint32_t neg = (p >> 30) & 1u;
 #pragma unroll
-    for (int g = 0; g < G; ++g) s[g] += R::flip(v.v[g], neg) * x.v[g];
+  for (int u = 0; u < 4; ++u) {
+    const uint32_t neg = (p[u] >> 30) & 1u;
+#pragma unroll
+    for (int g = 0; g < G; ++g) s[g] += R::flip(v[u].v[g], neg) * x[u].v[g];
   }
 }
 
@@ -379,13 +355,16 @@ __device__ inline void ab_entries(const uint32_t* __restrict__ ep, int n, const 
 // G-vectors, each lane walks its ELL entries for that chunk (the lane -> column map is per
 // chunk, so lanes of a group carry lists of similar length), and the chunk's sums are
 // added into sigma before the next chunk (one owner per element: a column belongs to one
-// tile).
+// tile).  A wave's entries of a chunk are one stream (its groups one after another, four
+// rows per batch); the next batch is loaded while the current one is used, across group
+// boundaries, and after the last batch the stream's first batch is loaded again for the
+// next ket rows.
 template <class T, int M>
 __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
                                              const int64_t* __restrict__ rpSa,
                                              const int32_t* __restrict__ colSa,
                                              const uint32_t* __restrict__ excSa,
-                                             const int64_t* __restrict__ eoff, const int32_t* __restrict__ eK,
+                                             const int64_t* __restrict__ soff, const int32_t* __restrict__ nq,
                                              const uint32_t* __restrict__ ent,
                                              const int32_t* __restrict__ colperm, int nch, int64_t chunk,
                                              int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
@@ -401,15 +380,30 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
   const int NT = blockDim.x;
   const int64_t npos = groups * 64;
   const int64_t ia = blockIdx.x / ntiles;
-  const int64_t col0 = (blockIdx.x - ia * ntiles) * (int64_t)NT * M;
+  const int64_t tile = blockIdx.x - ia * ntiles;
+  const int64_t col0 = tile * (int64_t)NT * M;
   const int lane = threadIdx.x & 63;
-  const int64_t gw = (col0 >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int64_t s0 = (tile * (NT >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * nch;  // wave-uniform
   if (threadIdx.x == 0)
     for (int g = 0; g < G; ++g) cbv[chunk].v[g] = T(0);
   const int64_t q0 = rpSa[ia], q1 = rpSa[ia + 1];
   for (int k = 0; k < nch; ++k) {
     const int64_t j0 = (int64_t)k * chunk;
-    const int64_t nj = min(nb, j0 + chunk) - j0;
+    const int nj = (int)(min(nb, j0 + chunk) - j0);
+    // this wave's entry stream of chunk k: `rows` rows (a multiple of four); pn holds the
+    // next batch, pf the row after it (wrapping to the start)
+    const int64_t so = soff[s0 + k];
+    const int rows = (int)((soff[s0 + k + 1] - so) >> 6);
+    const uint32_t* sp = ent + so;  // wave-uniform; lanes add their index
+    uint32_t pn[4];
+    int pf = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pn[u] = (uint32_t)chunk;
+    if (rows > 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pn[u] = sp[u * 64 + lane];
+      pf = rows > 4 ? 4 : 0;
+    }
     T acc[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) acc[m] = T(0);
@@ -434,45 +428,58 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
           for (int x = threadIdx.x; x < LL; x += NT) Vs[G * x + g] = T(0);
         }
       }
-      for (int64_t x0 = threadIdx.x; x0 < nj; x0 += 4 * NT) {
+      // the thread index, opaque here so that the fill's addresses are formed per step
+      // rather than held in registers through the products
+      int xt = (int)threadIdx.x;
+      asm volatile("" : "+v"(xt));
+      for (int x0 = xt; x0 < nj; x0 += 4 * NT) {
         RV v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int64_t x = x0 + (int64_t)u * NT;
+          const int x = x0 + u * NT;
 #pragma unroll
           for (int g = 0; g < G; ++g)
             v[u].v[g] = (has[g] && x < nj) ? __builtin_nontemporal_load(crow[g] + x) : T(0);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-          if (x0 + (int64_t)u * NT < nj) cbv[x0 + (int64_t)u * NT] = v[u];
+          if (x0 + u * NT < nj) cbv[x0 + u * NT] = v[u];
       }
       __syncthreads();
-      // the wave's group index, opaque to the compiler here: the per-group offsets and
-      // counts are then reloaded for every step instead of being held in 3M registers
-      int gws = (int)gw;
-      asm volatile("" : "+s"(gws));
+      // the stream's batch counts, opaque to the compiler here: they are then reloaded for
+      // every step instead of being held in M registers
+      int qi = (int)((s0 + k) * M);
+      asm volatile("" : "+s"(qi));
 #pragma unroll
       for (int m = 0; m < M; ++m) {
-        const int64_t g = (int64_t)gws + (int64_t)m * (NT >> 6);  // group of this wave's m-th columns
-        if (g < groups) {
-          const int64_t idx = g * nch + k;
-          const int64_t o = eoff[idx];
-          const int n = eK[idx];
-          T sg[G];
+        const int nb4 = nq[qi + m];  // batches of this wave's m-th group
+        T sg[G];
 #pragma unroll
-          for (int r = 0; r < G; ++r) sg[r] = T(0);
-          ab_entries<T>(ent + o + lane, n, cbv, Vv, sg);
-          T a = acc[m];
+        for (int r = 0; r < G; ++r) sg[r] = T(0);
+        for (int b = 0; b < nb4; ++b) {
+          uint32_t p[4];
+          const uint32_t* rp = sp + pf * 64;
 #pragma unroll
-          for (int r = 0; r < G; ++r) a += coef[r] * sg[r];
-          acc[m] = a;
+          for (int u = 0; u < 4; ++u) {
+            p[u] = pn[u];
+            pn[u] = rp[u * 64 + lane];
+          }
+          pf = pf + 4 == rows ? 0 : pf + 4;
+          ab_batch<T>(p, cbv, Vv, sg);
         }
+        T a = acc[m];
+#pragma unroll
+        for (int r = 0; r < G; ++r) a += coef[r] * sg[r];
+        asm volatile("" : "+v"(a));  // formed here, so the group's partial sums die here
+        acc[m] = a;
       }
     }
+    // opaque again: the column map is read here, not before the products
+    int ft = (int)threadIdx.x;
+    asm volatile("" : "+v"(ft));
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      const int64_t pos = col0 + (int64_t)m * NT + threadIdx.x;
+      const int64_t pos = col0 + (int64_t)m * NT + ft;
       if ((pos >> 6) < groups) {
         const int32_t ib = colperm[(int64_t)k * npos + pos];
         if (ib >= 0) sigma[ia * nb + ib] += acc[m];
@@ -578,7 +585,7 @@ void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* si
   const int64_t nblk = E.ntiles * P.na;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
   hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.nb, P.L, E.ntiles,
-                     P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.off.p, E.K.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
+                     P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.soff.p, E.nq.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
                      E.groups, eri, c, sigma);
 }
 

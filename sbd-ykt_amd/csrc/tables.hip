@@ -601,11 +601,25 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
       std::stable_sort(cols.begin(), cols.end(), [&](int32_t x, int32_t y) {
         return cnt[(size_t)x * nch + k] > cnt[(size_t)y * nch + k];
       });
-      for (size_t i = 0; i < cols.size(); ++i) perm[(size_t)k * npos + lo + i] = cols[i];
+      // groups of 64 columns in count order; each goes to the wave with the fewest rows so
+      // far that still has a free slot (the kernel walks slot j of wave w as group w + j*W)
+      const int Wv = e.nt / 64;
+      std::vector<int> load(Wv, 0), used(Wv, 0);
+      for (int j = 0; j < Wv * e.m; ++j) {
+        const size_t i0 = (size_t)j * 64;
+        const int mx = i0 < cols.size() ? cnt[(size_t)cols[i0] * nch + k] : 0;  // first lane holds the most
+        int best = -1;
+        for (int w = 0; w < Wv; ++w)
+          if (used[w] < e.m && (best < 0 || load[w] < load[best])) best = w;
+        const int64_t slot = (int64_t)best + (int64_t)used[best] * Wv;
+        ++used[best];
+        load[best] += (mx + 3) & ~3;
+        for (size_t l = 0; l < 64 && i0 + l < cols.size(); ++l) perm[(size_t)k * npos + lo + slot * 64 + l] = cols[i0 + l];
+      }
     }
   }
+  // rows of each (group, chunk) block: the longest list of its lanes, rounded up to four
   std::vector<int32_t> K((size_t)std::max<int64_t>(ng, 1), 0);
-  std::vector<int64_t> off((size_t)ng + 1, 0);
   for (int64_t g = 0; g < e.groups; ++g)
     for (int k = 0; k < nch; ++k) {
       int mx = 0;
@@ -613,10 +627,27 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
         const int32_t ib = perm[(size_t)k * npos + g * 64 + l];
         if (ib >= 0) mx = std::max(mx, cnt[(size_t)ib * nch + k]);
       }
-      K[g * nch + k] = mx;
-      off[g * nch + k + 1] = off[g * nch + k] + (int64_t)mx * 64;
+      K[g * nch + k] = (mx + 3) & ~3;
     }
-  const int64_t total = off[ng];
+  // one stream per (tile, wave, chunk): the wave's groups j = 0..m-1 one after another
+  const int W = e.nt / 64;
+  const int64_t nstream = e.ntiles * W * nch;
+  std::vector<int64_t> soff((size_t)nstream + 1, 0), off((size_t)std::max<int64_t>(ng, 1), 0);
+  std::vector<int32_t> nq((size_t)std::max<int64_t>(nstream * e.m, 1), 0);
+  for (int64_t t = 0; t < e.ntiles; ++t)
+    for (int w = 0; w < W; ++w)
+      for (int k = 0; k < nch; ++k) {
+        const int64_t s = (t * W + w) * nch + k;
+        int64_t o = soff[s];
+        for (int j = 0; j < e.m; ++j) {
+          const int64_t g = t * W * e.m + w + (int64_t)j * W;
+          off[g * nch + k] = o;
+          nq[s * e.m + j] = K[g * nch + k] / 4;
+          o += (int64_t)K[g * nch + k] * 64;
+        }
+        soff[s + 1] = o;
+      }
+  const int64_t total = soff[nstream];
   const uint32_t pad = (uint32_t)chunk;  // zero slot, pair 0
   std::vector<uint32_t> ent((size_t)std::max<int64_t>(total, 1), pad);
   // pass groups of the kernel's 16-byte LDS reads, as measured on gfx950 (the alpha-beta
@@ -645,10 +676,10 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
     }
   e.colperm.alloc(perm.size());
   SBDG_CHECK(hipMemcpy(e.colperm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
-  e.K.alloc(K.size());
-  SBDG_CHECK(hipMemcpy(e.K.p, K.data(), sizeof(int32_t) * K.size(), hipMemcpyHostToDevice));
-  e.off.alloc(off.size());
-  SBDG_CHECK(hipMemcpy(e.off.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
+  e.nq.alloc(nq.size());
+  SBDG_CHECK(hipMemcpy(e.nq.p, nq.data(), sizeof(int32_t) * nq.size(), hipMemcpyHostToDevice));
+  e.soff.alloc(soff.size());
+  SBDG_CHECK(hipMemcpy(e.soff.p, soff.data(), sizeof(int64_t) * soff.size(), hipMemcpyHostToDevice));
   e.ent.alloc(ent.size());
   SBDG_CHECK(hipMemcpy(e.ent.p, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
 }
