@@ -18,7 +18,9 @@
 // vector x is basis vector 0, and its sigma is rebuilt in FP64.  That gives the exact
 // Rayleigh quotient and residual of x (the only quantities used for convergence and
 // reported), an exact first row/column of the projected matrix, and the exact residual
-// drives the next correction vector.
+// drives the next correction vector.  With refresh_k <= 0 the refreshes follow the
+// residual instead: one when the FP32 estimate has fallen tenfold since the last exact
+// residual, or is below eps; a full basis then restarts on its FP32 sigma alone.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -483,6 +485,14 @@ void davidson(sbdg_ctx* c) {
   // matrix is all FP32-sigma entries
   bool have_anchor = false;
   double prev_exact = NAN;
+  // mixed with refresh_k <= 0: refresh when the FP32 residual estimate has fallen by
+  // ADAPT_DROP since the last exact residual (the first estimate before any), when it
+  // is below eps (to confirm convergence), or after ADAPT_MAX steps; a full basis
+  // restarts without a refresh
+  const bool adaptive = mixed && o.refresh_k <= 0;
+  constexpr double ADAPT_DROP = 0.1;
+  constexpr int ADAPT_MAX = 32;
+  double r_ref = -1.0;
   while (!converged && !stop && steps < max_steps) {
     const double ts = now();
     // sigma of the newest basis vector and column h of the projected matrix
@@ -508,10 +518,15 @@ void davidson(sbdg_ctx* c) {
     for (int k = 0; k <= h; ++k) Hs[k + MAXSUB * h] = Hs[h + MAXSUB * k] = dots[k];
     ++h;
     ++since;
-    const bool do_refresh = mixed && (since >= std::max(1, o.refresh_k) || v == nbmax);
+    bool do_refresh = mixed && !adaptive && (since >= o.refresh_k || v == nbmax);
     const bool fuse = v < nbmax && !do_refresh && h <= 32;  // expansion follows directly
     if (fuse) ritz_fused(h);
     else ritz(h);
+    if (adaptive) {
+      if (r_ref < 0.0) r_ref = rnorm;
+      do_refresh = rnorm < o.eps || (since >= 2 && rnorm < ADAPT_DROP * r_ref) || since >= ADAPT_MAX;
+      if (do_refresh && !w_valid) ritz(h);  // the fused step formed no Ritz vector
+    }
     if (!mixed) {
       const double el = log_step(true);
       if (rnorm < o.eps) {
@@ -535,6 +550,19 @@ void davidson(sbdg_ctx* c) {
       if (!do_refresh) {
         const double el = log_step(false);
         if (check_time(el, ts)) break;
+        if (v == nbmax) {
+          // adaptive schedule, full basis: restart onto `keep` Ritz vectors with their
+          // FP32 sigma, no FP64 sigma; basis vector 0 changes, so no exact anchor row
+          ++cycle;
+          rotate_impl<double>(c->V.p, N, h, s.U, MAXSUB, keep, N, c->st);
+          rotate_impl<float>(c->HV32.p, N, h, s.U, MAXSUB, keep, N, c->st);
+          std::fill(Hs.begin(), Hs.end(), 0.0);
+          for (int k = 0; k < keep; ++k) Hs[k + MAXSUB * k] = E[k];
+          v = h = keep;
+          have_anchor = false;
+          expand_or_stop();
+          continue;
+        }
         if (fuse) finish_fused();
         else expand_or_stop();
         continue;
@@ -551,6 +579,7 @@ void davidson(sbdg_ctx* c) {
       ++cycle;
       refresh();
       have_anchor = true;
+      r_ref = rnorm;
       const double el = log_step(true);
       if (rnorm < o.eps) {
         converged = true;

@@ -314,14 +314,18 @@ def test_thick_restart_and_warm_start():
     assert abs(e3 - e2) < 1e-9 and s3.stats()["iterations"] <= 2
 
 
-def test_mixed_precision_energy():
+@pytest.mark.parametrize("refresh_k", [4, 0])
+def test_mixed_precision_energy(refresh_k):
+    # refresh_k = 0: refreshes scheduled by the FP32 residual, restarts without refresh
     case = medium_case()
     e64 = solver(case, max_it=100, max_nb=12, restart_keep=4, eps=1e-8).run()
-    s = solver(case, max_it=100, max_nb=12, restart_keep=4, eps=1e-6, precision=1, refresh_k=4)
+    s = solver(case, max_it=100, max_nb=12, restart_keep=4, eps=1e-6, precision=1, refresh_k=refresh_k)
     e = s.run()
     st = s.stats()
-    assert st["sigma_fp32"] > 0
+    assert st["sigma_fp32"] > 0 and st["converged"] == 1
     assert abs(e - e64) < 1e-6
+    t, traj = s.trajectory()
+    assert len(traj) >= 1 and abs(traj[-1] - e64) < 1e-6  # exact energies only
 
 
 def test_hellmann_feynman_backward_vs_finite_differences():
