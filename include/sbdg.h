@@ -61,6 +61,9 @@ typedef struct {
   double core_gib;    /* workspace cap in GiB for the per-core products (default 16); with
                          doubles==0 a spin whose 512-column pass does not fit uses the
                          excitation-table form, with doubles==2 it is an error */
+  int overlap;        /* 1 (default): the alpha-beta sigma kernel runs on a second stream
+                         beside the spin phases; 0: all sigma kernels in one stream, in
+                         order (sbdg_bench_sigma's per-phase times need 0) */
 } sbdg_options;
 
 typedef struct {
@@ -177,7 +180,9 @@ int sbdg_get_hii(sbdg_ctx *ctx, double *hii);
 int sbdg_sigma(sbdg_ctx *ctx, const double *c, double *sigma, int precision);
 /* Times `reps` sigma builds on resident device vectors with HIP events on the context
  * stream.  ms[0] total per build; ms[1..5] per kernel phase (transpose, beta product,
- * alpha product + diagonal, transpose-add, alpha-beta). */
+ * alpha product + diagonal, transpose-add, alpha-beta) when options.overlap is 0.  With
+ * overlap, ms[1..3] are the spin phases as they run beside the alpha-beta kernel, ms[4]
+ * the wait for it after them and ms[5] the joining transpose-add. */
 int sbdg_bench_sigma(sbdg_ctx *ctx, int precision, int reps, double *ms);
 
 #ifdef __cplusplus

@@ -44,6 +44,8 @@ constexpr size_t LDS_BUDGET = 160 * 1024;  // alpha-beta kernel: all of a CU's L
 struct sbdg_ctx {
   int device = 0;
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;  // alpha-beta kernel, concurrent with the spin phases
+  hipEvent_t fork = nullptr, join = nullptr;
   std::string err;
   sbdg_options opt{};
   bool have_ints = false, have_strs = false, built = false, solved = false, have_rdm = false,
@@ -140,6 +142,9 @@ SigmaPlan plan(sbdg_ctx* c, bool fp32) {
   P.npair = (int64_t)c->L * (c->L - 1) / 2;
   P.Z = c->Zbuf.p;
   P.zbytes = c->Zbuf.n * sizeof(double);
+  P.st2 = c->opt.overlap ? c->st2 : nullptr;
+  P.fork = c->fork;
+  P.join = c->join;
   return P;
 }
 
@@ -233,13 +238,13 @@ void ensure_built(sbdg_ctx* c) {
 // device scratch for one sigma of precision T
 void sigma64(sbdg_ctx* c, const double* x, double* y, hipEvent_t* ev = nullptr) {
   const int64_t N = (int64_t)c->a.size() * c->b.size();
-  c->scr64.alloc(std::max<int64_t>(2 * N, 1));
+  c->scr64.alloc(std::max<int64_t>(3 * N, 1));
   sigma_apply<double>(plan(c, false), x, y, c->scr64.p, c->st, ev);
 }
 
 void sigma32(sbdg_ctx* c, const float* x, float* y, hipEvent_t* ev = nullptr) {
   const int64_t N = (int64_t)c->a.size() * c->b.size();
-  c->scr32.alloc(std::max<int64_t>(2 * N, 1));
+  c->scr32.alloc(std::max<int64_t>(3 * N, 1));
   sigma_apply<float>(plan(c, true), x, y, c->scr32.p, c->st, ev);
 }
 
@@ -699,7 +704,13 @@ sbdg_ctx* sbdg_create(int device) {
   if (hipSetDevice(device) != hipSuccess) return nullptr;
   auto* c = new sbdg_ctx();
   c->device = device;
-  if (hipStreamCreate(&c->st) != hipSuccess) {
+  if (hipStreamCreate(&c->st) != hipSuccess || hipStreamCreate(&c->st2) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
+    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return nullptr;
   }
@@ -711,6 +722,10 @@ void sbdg_destroy(sbdg_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
+  (void)hipStreamSynchronize(c->st2);
+  (void)hipEventDestroy(c->fork);
+  (void)hipEventDestroy(c->join);
+  (void)hipStreamDestroy(c->st2);
   (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -736,6 +751,7 @@ int sbdg_default_options(sbdg_options* o) {
   o->seed = 12345;
   o->doubles = 0;
   o->core_gib = 16.0;
+  o->overlap = 1;
   return SBDG_OK;
 }
 
@@ -816,6 +832,7 @@ int sbdg_set_options(sbdg_ctx* c, const sbdg_options* o) {
     if (!(o->eps >= 0.0)) return fail(c, SBDG_ERR_ARG, "eps must be >= 0");
     if (o->doubles < 0 || o->doubles > 2) return fail(c, SBDG_ERR_ARG, "doubles must be 0, 1 or 2");
     if (!(o->core_gib > 0.0)) return fail(c, SBDG_ERR_ARG, "core_gib must be > 0");
+    if (o->overlap != 0 && o->overlap != 1) return fail(c, SBDG_ERR_ARG, "overlap must be 0 or 1");
     if (c->built && (o->doubles != c->opt.doubles || o->core_gib != c->opt.core_gib)) c->built = false;
     c->opt = *o;
     return SBDG_OK;

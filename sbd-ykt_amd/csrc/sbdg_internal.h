@@ -107,8 +107,9 @@ struct AbTable {
 // multiple of four rows of 64 lane-interleaved entries (nq[stream*m + j] batches of four
 // rows) padded with a zero-weight entry: the wave reads its stream coalesced, four rows
 // at a time, with the next four rows in flight across group boundaries.
-// Entry: bits 0-17 ket column offset inside the chunk (the value `chunk` addresses a zero
-// slot: padding), bits 18-29 orbital pair b*L+j, bit 30 sign.  Within a lane the entries
+// Entry: bits 0-17 LDS byte address of the ket vector (16 * (L*L + column offset inside
+// the chunk); offset `chunk` is a zero vector: padding), bits 18-29 orbital pair b*L+j
+// (its slice vector sits at byte 16 * (b*L+j)), bit 31 sign.  Within a lane the entries
 // are ordered so that lanes read distinct LDS slots where possible.
 struct EllTable {
   int nch = 0;
@@ -184,8 +185,10 @@ struct SigmaPlan {
   int64_t npair = 0;
   void* Z = nullptr;    // core-product workspace of zbytes
   size_t zbytes = 0;
+  hipStream_t st2 = nullptr;  // when set: the alpha-beta terms run there, beside the spin phases
+  hipEvent_t fork = nullptr, join = nullptr;
 };
-// sigma = H c in the separated convention; scratch holds 2*na*nb elements of T.
+// sigma = H c in the separated convention; scratch holds 3*na*nb elements of T.
 template <class T>
 void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream_t st,
                  hipEvent_t* ev = nullptr);

@@ -279,7 +279,7 @@ constexpr int TT = 64;  // transpose tile
 // out[c][r] = in[r][c]   (ADD: out[c][r] += in[r][c])
 template <class T, bool ADD>
 __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T* __restrict__ in,
-                                                   T* __restrict__ out) {
+                                                   T* __restrict__ out, const T* __restrict__ add) {
   __shared__ T tile[TT][TT + 1];
   const int64_t r0 = (int64_t)blockIdx.y * TT, c0 = (int64_t)blockIdx.x * TT;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
   for (int k = ty; k < TT; k += 4) {
     const int64_t c = c0 + k, r = r0 + tx;
     if (r < R && c < C) {
-      if (ADD) out[c * R + r] += tile[tx][k];
+      if (ADD) out[c * R + r] += tile[tx][k] + (add ? add[c * R + r] : T(0));
       else out[c * R + r] = tile[tx][k];
     }
   }
@@ -300,21 +300,22 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
 // alpha-beta and cross-spin single terms.  One workgroup per (bra row ia, tile of lane
 // positions); thread t owns positions col0 + t + m*blockDim.x, m < M (every wave owns
 // whole 64-lane groups of the ELL table), i.e. columns colperm[position].  The ket rows
-// The ja in S(ia) are taken G at a time (G = 2 in FP64, 4 in FP32: 16 bytes either way):
+// ja in S(ia) are taken G at a time (G = 2 in FP64, 4 in FP32: 16 bytes either way):
 // the G (ai|..) slices and the G rows' current column chunk sit in LDS interleaved, so one
 // 16-byte LDS read serves all G rows, and every ELL entry read from the L2-resident table
-// serves them too.  LDS: [(chunk + 1 zero slot) G-vectors] [L*L slice G-vectors].
+// serves them too.  LDS: [L*L slice G-vectors] [chunk + 1 ket G-vectors, the last one zero].
 template <class T>
 struct AbRows {
   static constexpr int G = 16 / (int)sizeof(T);
   struct __align__(16) V {
     T v[G];
   };
-  __device__ static inline T flip(T x, uint32_t neg) {
+  // x with its sign flipped where bit 31 of sbit is set
+  __device__ static inline T flip(T x, uint32_t sbit) {
     if constexpr (sizeof(T) == 8)
-      return __longlong_as_double(__double_as_longlong(x) ^ ((unsigned long long)neg << 63));
+      return __longlong_as_double(__double_as_longlong(x) ^ ((unsigned long long)(sbit & 0x80000000u) << 32));
     else
-      return __int_as_float(__float_as_int(x) ^ (int)(neg << 31));
+      return __int_as_float(__float_as_int(x) ^ (int)(sbit & 0x80000000u));
   }
 };
 
@@ -326,25 +327,36 @@ __device__ inline void ab_slice(int ai, int64_t LL, const T* __restrict__ eri, T
   for (int x = threadIdx.x; x < LL; x += blockDim.x) V[G * x + g] = src[x];
 }
 
+// The LDS vector at byte address a of the workgroup's LDS.  The alpha-beta kernel has no
+// static LDS, so its dynamic LDS starts at address 0 (checked at launch) and entries carry
+// absolute byte addresses.
+typedef unsigned int lds_u32x4 __attribute__((ext_vector_type(4)));
+template <class V>
+__device__ inline V lds_at(uint32_t a) {
+  static_assert(sizeof(V) == 16, "16-byte LDS vectors");
+  const lds_u32x4 w = *reinterpret_cast<const __attribute__((address_space(3))) lds_u32x4*>(static_cast<size_t>(a));
+  V r;
+  __builtin_memcpy(&r, &w, 16);
+  return r;
+}
+
 // One batch of four stream rows of one lane: s[g] += sign_e * V[pair_e][g] * cb[jb_e][g] for
 // the G staged rows at once (all eight LDS reads issued before the products).
 template <class T>
-__device__ inline void ab_batch(const uint32_t (&p)[4], const typename AbRows<T>::V* cbv,
-                                const typename AbRows<T>::V* Vv, T (&s)[AbRows<T>::G]) {
+__device__ inline void ab_batch(const uint32_t (&p)[4], T (&s)[AbRows<T>::G]) {
   using R = AbRows<T>;
   using RV = typename R::V;
   constexpr int G = R::G;
   RV v[4], x[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    v[u] = Vv[(p[u] >> 18) & 0xFFFu];
-    x[u] = cbv[p[u] & 0x3FFFFu];
+    v[u] = lds_at<RV>((p[u] >> 14) & 0xFFF0u);  // slice vector of pair b*L+j
+    x[u] = lds_at<RV>(p[u] & 0x3FFFFu);         // ket vector
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const uint32_t neg = (p[u] >> 30) & 1u;
 #pragma unroll
-    for (int g = 0; g < G; ++g) s[g] += R::flip(v[u].v[g], neg) * x[u].v[g];
+    for (int g = 0; g < G; ++g) s[g] += R::flip(v[u].v[g], p[u]) * x[u].v[g];
   }
 }
 
@@ -368,13 +380,13 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
                                              const uint32_t* __restrict__ ent,
                                              const int32_t* __restrict__ colperm, int nch, int64_t chunk,
                                              int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
-                                             T* __restrict__ sigma) {
+                                             T* __restrict__ sigma, int assign) {
   using R = AbRows<T>;
   using RV = typename R::V;
   constexpr int G = R::G;
   extern __shared__ __align__(16) unsigned char smem[];
-  RV* cbv = reinterpret_cast<RV*>(smem);  // chunk + 1 G-vectors (the last one stays zero: padding)
-  RV* Vv = cbv + chunk + 1;               // L*L G-vectors
+  RV* Vv = reinterpret_cast<RV*>(smem);  // L*L G-vectors
+  RV* cbv = Vv + (int64_t)L * L;          // chunk + 1 G-vectors (the last one stays zero: padding)
   T* Vs = reinterpret_cast<T*>(Vv);
   const int64_t LL = (int64_t)L * L;
   const int NT = blockDim.x;
@@ -465,7 +477,7 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
             pn[u] = rp[u * 64 + lane];
           }
           pf = pf + 4 == rows ? 0 : pf + 4;
-          ab_batch<T>(p, cbv, Vv, sg);
+          ab_batch<T>(p, sg);
         }
         T a = acc[m];
 #pragma unroll
@@ -482,7 +494,10 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
       const int64_t pos = col0 + (int64_t)m * NT + ft;
       if ((pos >> 6) < groups) {
         const int32_t ib = colperm[(int64_t)k * npos + pos];
-        if (ib >= 0) sigma[ia * nb + ib] += acc[m];
+        if (ib >= 0) {
+          if (assign && k == 0) sigma[ia * nb + ib] = acc[m];  // every column has a position in chunk 0
+          else sigma[ia * nb + ib] += acc[m];
+        }
       }
     }
   }
@@ -573,37 +588,43 @@ void launch_spin(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, c
 }
 
 template <class T, bool ADD>
-void launch_transpose(int64_t R, int64_t C, const T* in, T* out, hipStream_t st) {
+void launch_transpose(int64_t R, int64_t C, const T* in, T* out, hipStream_t st, const T* add = nullptr) {
   if (R == 0 || C == 0) return;
   dim3 g((unsigned)((C + TT - 1) / TT), (unsigned)((R + TT - 1) / TT));
-  hipLaunchKernelGGL((k_transpose<T, ADD>), g, dim3(256), 0, st, R, C, in, out);
+  hipLaunchKernelGGL((k_transpose<T, ADD>), g, dim3(256), 0, st, R, C, in, out, add);
 }
 
 template <class T, int M>
-void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* sigma, hipStream_t st) {
+void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* sigma, hipStream_t st, int assign) {
   const EllTable& E = *P.ell;
   const int64_t nblk = E.ntiles * P.na;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
+  static const bool lds_at_zero = [] {  // entries hold absolute LDS byte addresses
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_ab<T, M>)) == hipSuccess &&
+           fa.sharedSizeBytes == 0;
+  }();
+  if (!lds_at_zero) throw Error(SBDG_ERR_STATE, "alpha-beta kernel has static LDS: entry addresses invalid");
   hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.nb, P.L, E.ntiles,
                      P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.soff.p, E.nq.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
-                     E.groups, eri, c, sigma);
+                     E.groups, eri, c, sigma, assign);
 }
 
 template <class T>
-void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream_t st) {
+void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream_t st, int assign) {
   if (P.na == 0 || P.nb == 0) return;
   const EllTable& E = *P.ell;
   if ((int64_t)E.nt * E.m * E.ntiles != E.groups * 64) throw Error(SBDG_ERR_STATE, "alpha-beta table geometry");
   const size_t lds = ab_lds_bytes(P.L, E.chunk);
   switch (E.m) {  // columns per thread, fixed when the table was built
-    case 1: launch_ab_m<T, 1>(P, lds, eri, c, sigma, st); break;
-    case 2: launch_ab_m<T, 2>(P, lds, eri, c, sigma, st); break;
-    case 4: launch_ab_m<T, 4>(P, lds, eri, c, sigma, st); break;
-    case 8: launch_ab_m<T, 8>(P, lds, eri, c, sigma, st); break;
-    case 12: launch_ab_m<T, 12>(P, lds, eri, c, sigma, st); break;
-    case 16: launch_ab_m<T, 16>(P, lds, eri, c, sigma, st); break;
-    case 20: launch_ab_m<T, 20>(P, lds, eri, c, sigma, st); break;
-    case 24: launch_ab_m<T, 24>(P, lds, eri, c, sigma, st); break;
+    case 1: launch_ab_m<T, 1>(P, lds, eri, c, sigma, st, assign); break;
+    case 2: launch_ab_m<T, 2>(P, lds, eri, c, sigma, st, assign); break;
+    case 4: launch_ab_m<T, 4>(P, lds, eri, c, sigma, st, assign); break;
+    case 8: launch_ab_m<T, 8>(P, lds, eri, c, sigma, st, assign); break;
+    case 12: launch_ab_m<T, 12>(P, lds, eri, c, sigma, st, assign); break;
+    case 16: launch_ab_m<T, 16>(P, lds, eri, c, sigma, st, assign); break;
+    case 20: launch_ab_m<T, 20>(P, lds, eri, c, sigma, st, assign); break;
+    case 24: launch_ab_m<T, 24>(P, lds, eri, c, sigma, st, assign); break;
     default: throw Error(SBDG_ERR_STATE, "alpha-beta columns per thread");
   }
 }
@@ -634,6 +655,15 @@ void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream
     eri = reinterpret_cast<const T*>(P.eri32);
   }
   if (ev) SBDG_CHECK(hipEventRecord(ev[0], st));
+  // the alpha-beta terms (LDS-bound) into their own buffer on the second stream, while the
+  // spin phases (HBM-bound) run on this one; the transpose-add joins them
+  T* sab = scratch + 2 * N;
+  if (P.st2) {
+    SBDG_CHECK(hipEventRecord(P.fork, st));
+    SBDG_CHECK(hipStreamWaitEvent(P.st2, P.fork, 0));
+    launch_ab<T>(P, eri, c, sab, P.st2, 1);
+    SBDG_CHECK(hipEventRecord(P.join, P.st2));
+  }
   launch_transpose<T, false>(P.na, P.nb, c, cT, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[1], st));
   const T* W;
@@ -652,10 +682,17 @@ void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream
   if (ev) SBDG_CHECK(hipEventRecord(ev[2], st));
   launch_spin<T>(true, P.na, P.nb, *P.ma, Jb, P.ca, W, P.npair, c, hii, sigma, Z, P, st);
   if (ev) SBDG_CHECK(hipEventRecord(ev[3], st));
-  launch_transpose<T, true>(P.nb, P.na, sT, sigma, st);
-  if (ev) SBDG_CHECK(hipEventRecord(ev[4], st));
-  launch_ab<T>(P, eri, c, sigma, st);
-  if (ev) SBDG_CHECK(hipEventRecord(ev[5], st));
+  if (P.st2) {
+    SBDG_CHECK(hipStreamWaitEvent(st, P.join, 0));
+    if (ev) SBDG_CHECK(hipEventRecord(ev[4], st));
+    launch_transpose<T, true>(P.nb, P.na, sT, sigma, st, sab);
+    if (ev) SBDG_CHECK(hipEventRecord(ev[5], st));
+  } else {
+    launch_transpose<T, true>(P.nb, P.na, sT, sigma, st);
+    if (ev) SBDG_CHECK(hipEventRecord(ev[4], st));
+    launch_ab<T>(P, eri, c, sigma, st, 0);
+    if (ev) SBDG_CHECK(hipEventRecord(ev[5], st));
+  }
   SBDG_CHECK(hipGetLastError());
 }
 

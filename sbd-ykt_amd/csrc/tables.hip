@@ -495,9 +495,11 @@ static void ab_geometry(int64_t nb, int& nt, int& m) {
 // broadcast and cost nothing.  Padding (the zero slot, pair 0) may be placed at any step
 // while a lane has slack.  Greedy: at each step the lanes with least slack choose first,
 // each taking the entry that adds the fewest new addresses to slots already in use.
-// lists[l] holds lane l's entries (ket offset | pair << 18 | sign << 30); out gets K*64.
-static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int P, int S, uint32_t vbase,
-                           uint32_t pad, uint32_t* out) {
+// lists[l] holds lane l's entries (ket byte address | pair << 18 | sign << 31); out gets
+// K*64.  Slots count 16-byte LDS vectors: the pair's slice vector is the pair index, the
+// ket's is its byte address / 16.
+static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int P, int S, uint32_t pad,
+                           uint32_t* out) {
   struct Use {
     uint32_t addr[8];
     int n;
@@ -526,7 +528,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
         int best = -1, bc = 1 << 30;
         for (int j = 0; j < rem[l]; ++j) {
           const uint32_t w = ls[j];
-          const uint32_t ca = w & 0x3FFFFu, va = vbase + ((w >> 18) & 0xFFFu);
+          const uint32_t ca = (w & 0x3FFFFu) >> 4, va = (w >> 18) & 0xFFFu;
           const int cst = cost1(uc[ca % S], ca) + cost1(uv[va % S], va);
           if (cst < bc) {
             bc = cst;
@@ -536,7 +538,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
         }
         uint32_t w = pad;
         if (slack > 0) {
-          const uint32_t ca = pad & 0x3FFFFu, va = vbase + ((pad >> 18) & 0xFFFu);
+          const uint32_t ca = (pad & 0x3FFFFu) >> 4, va = (pad >> 18) & 0xFFFu;
           const int cst = cost1(uc[ca % S], ca) + cost1(uv[va % S], va);
           if (best < 0 || cst < bc) best = -1;
         }
@@ -545,7 +547,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
           ls[best] = ls[rem[l] - 1];  // order of the rest does not matter
           --rem[l];
         }
-        const uint32_t ca = w & 0x3FFFFu, va = vbase + ((w >> 18) & 0xFFFu);
+        const uint32_t ca = (w & 0x3FFFFu) >> 4, va = (w >> 18) & 0xFFFu;
         add1(uc[ca % S], ca);
         add1(uv[va % S], va);
         out[(size_t)e * 64 + l] = w;
@@ -556,7 +558,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
 
 void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st) {
   const int64_t nb = sb.n;
-  if (chunk >= (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta chunk beyond 2^18 columns");
+  if (16 * ((int64_t)L * L + chunk + 1) > (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta LDS beyond 256 KiB");
   if (L > 64) throw Error(SBDG_ERR_LIMIT, "norb beyond 64");
   e.chunk = chunk;
   e.nch = (int)std::max<int64_t>(1, (nb + chunk - 1) / chunk);
@@ -648,13 +650,13 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
         soff[s + 1] = o;
       }
   const int64_t total = soff[nstream];
-  const uint32_t pad = (uint32_t)chunk;  // zero slot, pair 0
+  const uint32_t pad = ((uint32_t)(L * L) + (uint32_t)chunk) << 4;  // zero slot, pair 0
   std::vector<uint32_t> ent((size_t)std::max<int64_t>(total, 1), pad);
   // pass groups of the kernel's 16-byte LDS reads, as measured on gfx950 (the alpha-beta
   // kernel at norb=36, 20,000^2 with four candidate models; this one was fastest): 32 lanes
   // over 16 slots of 16 bytes (256 bytes of banks)
   const int P = elem >= 16 ? 32 : 16, S = 16;
-  const uint32_t vbase = (uint32_t)(chunk + 1);  // integral pairs follow the ket pairs
+  const uint32_t kbase = (uint32_t)(L * L);  // ket vectors follow the L*L slice vectors
   std::vector<std::vector<uint32_t>> lists(64);
   for (int64_t g = 0; g < e.groups; ++g)
     for (int k = 0; k < nch; ++k) {
@@ -669,10 +671,10 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
           if (col[q] < j0 || col[q] >= j0 + chunk) continue;
           const uint32_t x = exc[q];
           const uint32_t j = x & 63u, b = (x >> 6) & 63u, neg = (x >> 24) & 1u;
-          lst.push_back((uint32_t)(col[q] - j0) | ((b * (uint32_t)L + j) << 18) | (neg << 30));
+          lst.push_back(((kbase + (uint32_t)(col[q] - j0)) << 4) | ((b * (uint32_t)L + j) << 18) | (neg << 31));
         }
       }
-      schedule_block(lists, K[g * nch + k], P, S, vbase, pad, ent.data() + base);
+      schedule_block(lists, K[g * nch + k], P, S, pad, ent.data() + base);
     }
   e.colperm.alloc(perm.size());
   SBDG_CHECK(hipMemcpy(e.colperm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));

@@ -36,7 +36,7 @@ class Options(C.Structure):
         ("init", C.c_int), ("do_rdm", C.c_int), ("ratio", C.c_double), ("threshold", C.c_double),
         ("restart_keep", C.c_int), ("precision", C.c_int), ("refresh_k", C.c_int),
         ("target_de", C.c_double), ("verbose", C.c_int), ("seed", C.c_uint64),
-        ("doubles", C.c_int), ("core_gib", C.c_double),
+        ("doubles", C.c_int), ("core_gib", C.c_double), ("overlap", C.c_int),
     ]
 
 

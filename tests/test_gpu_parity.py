@@ -180,22 +180,27 @@ def test_sigma_c1_shape_row_sample(doubles, core_gib):
         assert np.abs(sx32[blk] - sx[blk]).max() <= 2e-5 * np.abs(sx[blk]).max()
 
 
-def test_sigma_wide_beta_set_vs_oracle():
+@pytest.mark.parametrize("overlap", [1, 0])
+def test_sigma_wide_beta_set_vs_oracle(overlap):
     """30,000 beta strings (norb=24, 5 electrons) against 12 alpha strings: the alpha-beta
     kernel then spans several column tiles (24,576 columns each), each with its own
-    per-chunk lane order."""
+    per-chunk lane order.  overlap=1 runs it on its own stream beside the spin phases,
+    overlap=0 after them on one stream."""
     rng = np.random.default_rng(77)
     L = 24
     h1, eri = syn.synthetic_integrals(L, rng)
     astr = syn.sample_strings(L, 5, 12, rng)
     bstr = syn.sample_strings(L, 5, 30000, rng)
     case = dict(norb=L, core=0.5, h1=h1, eri=eri, astr=astr, bstr=bstr)
-    s, p = solver(case), oproblem(case)
+    s, p = solver(case, overlap=overlap), oproblem(case)
     x = rng.normal(size=s.ndet)
     sx, ox = s.sigma(x), p.sigma(x)
     assert np.abs(sx - ox).max() <= 1e-12 * np.abs(ox).max()
     sx32 = s.sigma(x, precision=1)
     assert np.abs(sx32 - ox).max() <= 2e-5 * np.abs(ox).max()
+    ms = s.bench_sigma(0, 2)
+    assert ms[0] > 0 and abs(ms[1:].sum() - ms[0]) <= 1e-3 * ms[0] + 1e-3
+    assert np.array_equal(s.sigma(x), sx)  # repeatable
 
 
 def test_core_workspace_cap():
