@@ -7,7 +7,8 @@
  * Plain pointers and sizes only; the caller owns every host buffer and the library
  * copies in and out; device memory belongs to the context.  Every function returns
  * SBDG_OK (0) or a negative status; sbdg_last_error() gives the message.  One context
- * per host thread; calls on one context are serialised on its single HIP stream.
+ * per host thread; calls on one context are serialised on its HIP stream (sigma's
+ * alpha-beta kernel runs on a second stream of the context, joined before sigma returns).
  *
  * Conventions at this boundary are the reference's:
  *  - a string is a uint64 whose bit i is orbital i (bit_manipulation.h:158-172 reads
