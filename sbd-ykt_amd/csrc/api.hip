@@ -254,7 +254,7 @@ struct Small {  // device scalars carved from one buffer
 
 Small carve(sbdg_ctx* c) {
   const size_t ms = MAXSUB * MAXSUB;
-  c->small.alloc(4 * ms + 4 * MAXSUB + 64 + 1024 * (MAXSUB + 2));
+  c->small.alloc(4 * ms + 4 * MAXSUB + 64 + 1024 * (MAXSUB + 2));  // partials of up to 1024 reduction blocks
   Small s;
   s.H = c->small.p;
   s.U = s.H + ms;

@@ -13,7 +13,7 @@ namespace sbdg {
 namespace {
 
 constexpr int RT = 256;   // reduction block
-constexpr int RB = 1024;  // reduction blocks (upper bound)
+constexpr int RB = 1024;  // reduction blocks (upper bound; the partial-sum buffer holds RB rows)
 
 inline unsigned nblocks(int64_t n) {
   int64_t b = (n + RT * 4 - 1) / (RT * 4);
@@ -35,6 +35,34 @@ __device__ inline double block_sum(double v, double* sh) {
   double t = 0.0;
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
   return t;
+}
+
+// VEC consecutive elements at p as doubles (VEC == 2: p is 2-element aligned, one
+// 16-byte load for FP64, 8-byte for FP32), and the store back
+template <int VEC, class T>
+__device__ inline void ld_vec(const T* __restrict__ p, double (&o)[VEC]) {
+  if constexpr (VEC == 2) {
+    if constexpr (sizeof(T) == 8) {
+      const double2 v = *reinterpret_cast<const double2*>(p);
+      o[0] = v.x;
+      o[1] = v.y;
+    } else {
+      const float2 v = *reinterpret_cast<const float2*>(p);
+      o[0] = (double)v.x;
+      o[1] = (double)v.y;
+    }
+  } else {
+    o[0] = (double)p[0];
+  }
+}
+template <int VEC, class T>
+__device__ inline void st_vec(T* __restrict__ p, const double (&v)[VEC]) {
+  if constexpr (VEC == 2) {
+    if constexpr (sizeof(T) == 8) *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    else *reinterpret_cast<float2*>(p) = make_float2((float)v[0], (float)v[1]);
+  } else {
+    p[0] = (T)v[0];
+  }
 }
 
 // partial[blk*m + k] -> out[k], one block, fixed order
@@ -85,7 +113,7 @@ __global__ void k_f2d(const float* __restrict__ in, double* __restrict__ out, in
     out[t] = (double)in[t];
 }
 
-template <class TY, int MM>
+template <class TY, int MM, int VEC>
 __global__ void __launch_bounds__(RT) k_mdot(const double* __restrict__ X, int64_t ldx, int m,
                                              const TY* __restrict__ y, int64_t n,
                                              double* __restrict__ part) {
@@ -94,11 +122,18 @@ __global__ void __launch_bounds__(RT) k_mdot(const double* __restrict__ X, int64
 #pragma unroll
   for (int k = 0; k < MM; ++k) acc[k] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
-    const double yi = (double)y[i];
+  for (int64_t u = (int64_t)blockIdx.x * RT + threadIdx.x; u < n / VEC; u += stride) {
+    const int64_t i = u * VEC;
+    double yv[VEC];
+    ld_vec<VEC>(y + i, yv);
 #pragma unroll
     for (int k = 0; k < MM; ++k)
-      if (k < m) acc[k] += X[k * ldx + i] * yi;
+      if (k < m) {
+        double xv[VEC];
+        ld_vec<VEC>(X + k * ldx + i, xv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[k] += xv[e] * yv[e];
+      }
   }
 #pragma unroll
   for (int k = 0; k < MM; ++k) {
@@ -109,7 +144,7 @@ __global__ void __launch_bounds__(RT) k_mdot(const double* __restrict__ X, int64
   }
 }
 
-template <class TH, int MM>
+template <class TH, int MM, int VEC>
 __global__ void __launch_bounds__(RT) k_ritz(const double* __restrict__ V, int64_t ldv,
                                              const TH* __restrict__ HV, int64_t ldh, int m,
                                              const double* __restrict__ u,
@@ -123,19 +158,31 @@ __global__ void __launch_bounds__(RT) k_ritz(const double* __restrict__ V, int64
   const double th = theta[0];
   double sw = 0.0, sr = 0.0;
   const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
-    double w = 0.0, h = 0.0;
+  for (int64_t q = (int64_t)blockIdx.x * RT + threadIdx.x; q < n / VEC; q += stride) {
+    const int64_t i = q * VEC;
+    double w[VEC], h[VEC], r[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) w[e] = h[e] = 0.0;
 #pragma unroll
     for (int k = 0; k < MM; ++k)
       if (k < m) {
-        w += uk[k] * V[k * ldv + i];
-        h += uk[k] * (double)HV[k * ldh + i];
+        double v[VEC], hv[VEC];
+        ld_vec<VEC>(V + k * ldv + i, v);
+        ld_vec<VEC>(HV + k * ldh + i, hv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          w[e] += uk[k] * v[e];
+          h[e] += uk[k] * hv[e];
+        }
       }
-    const double r = h - th * w;
-    W[i] = w;
-    R[i] = r;
-    sw += w * w;
-    sr += r * r;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+      r[e] = h[e] - th * w[e];
+      sw += w[e] * w[e];
+      sr += r[e] * r[e];
+    }
+    st_vec<VEC>(W + i, w);
+    st_vec<VEC>(R + i, r);
   }
   sw = block_sum(sw, sh);
   sr = block_sum(sr, sh);
@@ -150,7 +197,7 @@ __global__ void __launch_bounds__(RT) k_ritz(const double* __restrict__ V, int64
 // t = r / (theta - hii) (regularised as k_precond; the missing 1/|r| is a common factor
 // that the normalisation after Gram-Schmidt removes) written to T, and the partials of
 // V_k . t (k < m), |r|^2 and |t|^2.  The Ritz vector itself is not formed.
-template <class TH, int MM>
+template <class TH, int MM, int VEC>
 __global__ void __launch_bounds__(RT) k_ritz_expand(const double* __restrict__ V, int64_t ldv,
                                                     const TH* __restrict__ HV, int64_t ldh, int m,
                                                     const double* __restrict__ u,
@@ -168,23 +215,41 @@ __global__ void __launch_bounds__(RT) k_ritz_expand(const double* __restrict__ V
   const double eps_reg = 1.0e-12;
   double sr = 0.0, st = 0.0;
   const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
-    double w = 0.0, h = 0.0;
+  for (int64_t q = (int64_t)blockIdx.x * RT + threadIdx.x; q < n / VEC; q += stride) {
+    const int64_t i = q * VEC;
+    double w[VEC], h[VEC], t[VEC], hd[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) w[e] = h[e] = 0.0;
 #pragma unroll
     for (int k = 0; k < MM; ++k)
       if (k < m) {
-        w += uk[k] * V[k * ldv + i];
-        h += uk[k] * (double)HV[k * ldh + i];
+        double v[VEC], hv[VEC];
+        ld_vec<VEC>(V + k * ldv + i, v);
+        ld_vec<VEC>(HV + k * ldh + i, hv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          w[e] += uk[k] * v[e];
+          h[e] += uk[k] * hv[e];
+        }
       }
-    const double r = h - th * w;
-    const double den = th - hii[i];
-    const double t = (fabs(den) > eps_reg) ? r / den : r / (den - eps_reg);
-    T[i] = t;
-    sr += r * r;
-    st += t * t;
+    ld_vec<VEC>(hii + i, hd);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+      const double r = h[e] - th * w[e];
+      const double den = th - hd[e];
+      t[e] = (fabs(den) > eps_reg) ? r / den : r / (den - eps_reg);
+      sr += r * r;
+      st += t[e] * t[e];
+    }
+    st_vec<VEC>(T + i, t);
 #pragma unroll
     for (int k = 0; k < MM; ++k)
-      if (k < m) d[k] += V[k * ldv + i] * t;
+      if (k < m) {
+        double v[VEC];
+        ld_vec<VEC>(V + k * ldv + i, v);  // the same elements as above: L2, usually registers
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) d[k] += v[e] * t[e];
+      }
   }
 #pragma unroll
   for (int k = 0; k < MM; ++k) {
@@ -201,10 +266,16 @@ __global__ void __launch_bounds__(RT) k_ritz_expand(const double* __restrict__ V
   }
 }
 
+template <int VEC>
 __global__ void k_scale(double* __restrict__ x, const double* __restrict__ n2, int64_t n) {
   const double f = 1.0 / sqrt(n2[0]);
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-    x[t] *= f;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n / VEC; q += (int64_t)gridDim.x * blockDim.x) {
+    double v[VEC];
+    ld_vec<VEC>(x + q * VEC, v);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) v[e] *= f;
+    st_vec<VEC>(x + q * VEC, v);
+  }
 }
 
 __global__ void k_precond(const double* __restrict__ R, const double* __restrict__ rn2,
@@ -220,7 +291,7 @@ __global__ void k_precond(const double* __restrict__ R, const double* __restrict
   }
 }
 
-template <int MM>
+template <int MM, int VEC>
 __global__ void __launch_bounds__(RT) k_gs(const double* __restrict__ V, int64_t ldv, int m,
                                            const double* __restrict__ s, double* __restrict__ t, int64_t n,
                                            double* __restrict__ part) {
@@ -230,31 +301,49 @@ __global__ void __launch_bounds__(RT) k_gs(const double* __restrict__ V, int64_t
   for (int k = 0; k < MM; ++k) sk[k] = (k < m) ? s[k] : 0.0;
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
-    double v = t[i];
+  for (int64_t q = (int64_t)blockIdx.x * RT + threadIdx.x; q < n / VEC; q += stride) {
+    const int64_t i = q * VEC;
+    double v[VEC];
+    ld_vec<VEC>(t + i, v);
 #pragma unroll
     for (int k = 0; k < MM; ++k)
-      if (k < m) v -= sk[k] * V[k * ldv + i];
-    t[i] = v;
-    acc += v * v;
+      if (k < m) {
+        double x[VEC];
+        ld_vec<VEC>(V + k * ldv + i, x);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) v[e] -= sk[k] * x[e];
+      }
+    st_vec<VEC>(t + i, v);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) acc += v[e] * v[e];
   }
   acc = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-template <class T, int MM>
+template <class T, int MM, int VEC>
 __global__ void k_rotate(T* __restrict__ X, int64_t ldx, int m, const double* __restrict__ U, int ld, int mk,
                          int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double x[MM];
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n / VEC; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q * VEC;
+    double x[MM][VEC];
 #pragma unroll
-    for (int k = 0; k < MM; ++k) x[k] = (k < m) ? (double)X[k * ldx + i] : 0.0;
+    for (int k = 0; k < MM; ++k) {
+      if (k < m) ld_vec<VEC>(X + k * ldx + i, x[k]);
+      else
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) x[k][e] = 0.0;
+    }
     for (int j = 0; j < mk; ++j) {
-      double y = 0.0;
+      double y[VEC];
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) y[e] = 0.0;
 #pragma unroll
       for (int k = 0; k < MM; ++k)
-        if (k < m) y += U[k + ld * j] * x[k];
-      X[j * ldx + i] = (T)y;
+        if (k < m)
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) y[e] += U[k + ld * j] * x[k][e];
+      st_vec<VEC>(X + j * ldx + i, y);
     }
   }
 }
@@ -380,6 +469,9 @@ void dispatch_m(int m, F&& f) {
   else throw Error(SBDG_ERR_LIMIT, "more than 64 subspace vectors");
 }
 
+// two elements per thread and 16-byte loads when every vector starts 2-element aligned
+inline bool pairs(int64_t n, int64_t ld1, int64_t ld2 = 0) { return n % 2 == 0 && ld1 % 2 == 0 && ld2 % 2 == 0; }
+
 }  // namespace
 
 void sign_transform(const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, const double* in, double* out,
@@ -409,7 +501,10 @@ void multi_dot_impl(const double* X, int64_t ldx, int m, const double* y, int64_
                     hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    hipLaunchKernelGGL((k_mdot<double, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+    if (pairs(n, ldx))
+      hipLaunchKernelGGL((k_mdot<double, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+    else
+      hipLaunchKernelGGL((k_mdot<double, decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
   });
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, m, out);
   SBDG_CHECK(hipGetLastError());
@@ -419,7 +514,10 @@ void multi_dot_impl(const double* X, int64_t ldx, int m, const float* y, int64_t
                     hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    hipLaunchKernelGGL((k_mdot<float, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+    if (pairs(n, ldx))
+      hipLaunchKernelGGL((k_mdot<float, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+    else
+      hipLaunchKernelGGL((k_mdot<float, decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
   });
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, m, out);
   SBDG_CHECK(hipGetLastError());
@@ -431,8 +529,12 @@ void ritz_combine_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, 
                        hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    hipLaunchKernelGGL((k_ritz<TH, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u, theta,
-                       W, R, n, part);
+    if (pairs(n, ldv, ldh))
+      hipLaunchKernelGGL((k_ritz<TH, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u,
+                         theta, W, R, n, part);
+    else
+      hipLaunchKernelGGL((k_ritz<TH, decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u,
+                         theta, W, R, n, part);
   });
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, 2, out);
   SBDG_CHECK(hipGetLastError());
@@ -449,9 +551,14 @@ void ritz_expand_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, i
   if (m > 32) throw Error(SBDG_ERR_LIMIT, "fused Ritz step beyond 32 vectors");
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    if constexpr (decltype(MM)::value <= 32)
-      hipLaunchKernelGGL((k_ritz_expand<TH, decltype(MM)::value>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u,
-                         theta, hii, T, n, part);
+    if constexpr (decltype(MM)::value <= 32) {
+      if (pairs(n, ldv, ldh))
+        hipLaunchKernelGGL((k_ritz_expand<TH, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh,
+                           m, u, theta, hii, T, n, part);
+      else
+        hipLaunchKernelGGL((k_ritz_expand<TH, decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh,
+                           m, u, theta, hii, T, n, part);
+    }
   });
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, m + 2, out);
   SBDG_CHECK(hipGetLastError());
@@ -465,7 +572,8 @@ template void ritz_expand_impl<float>(const double*, int64_t, const float*, int6
 
 void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_scale, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
+  if (pairs(n, 0)) hipLaunchKernelGGL(k_scale<2>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
+  else hipLaunchKernelGGL(k_scale<1>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
 }
 
 void precond_impl(const double* R, const double* rn2, const double* hii, const double* theta, double* t, int64_t n,
@@ -478,7 +586,10 @@ void gs_impl(const double* V, int64_t ldv, int m, const double* s, double* t, in
              hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    hipLaunchKernelGGL((k_gs<decltype(MM)::value>), dim3(g), dim3(RT), 0, st, V, ldv, m, s, t, n, part);
+    if (pairs(n, ldv))
+      hipLaunchKernelGGL((k_gs<decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, V, ldv, m, s, t, n, part);
+    else
+      hipLaunchKernelGGL((k_gs<decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, V, ldv, m, s, t, n, part);
   });
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, 1, out);
   SBDG_CHECK(hipGetLastError());
@@ -488,8 +599,12 @@ template <class T>
 void rotate_impl(T* X, int64_t ldx, int m, const double* U, int ld, int mk, int64_t n, hipStream_t st) {
   if (n == 0) return;
   dispatch_m<0>(m, [&](auto MM) {
-    hipLaunchKernelGGL((k_rotate<T, decltype(MM)::value>), dim3(nblocks(n) * 4), dim3(256), 0, st, X, ldx, m, U, ld,
-                       mk, n);
+    constexpr int mm = decltype(MM)::value;
+    if (mm <= 16 && pairs(n, ldx))  // x[MM][2] in registers
+      hipLaunchKernelGGL((k_rotate<T, mm, (mm <= 16 ? 2 : 1)>), dim3(nblocks(n) * 4), dim3(256), 0, st, X, ldx, m, U,
+                         ld, mk, n);
+    else
+      hipLaunchKernelGGL((k_rotate<T, mm, 1>), dim3(nblocks(n) * 4), dim3(256), 0, st, X, ldx, m, U, ld, mk, n);
   });
   SBDG_CHECK(hipGetLastError());
 }
