@@ -294,7 +294,8 @@ void initial_vector(sbdg_ctx* c, double* W) {
 // |t|^2 before projection at n0_dev; normalises t and returns |t| after projection (0 when
 // nothing is left).  A second pass runs only when the first removed most of t
 // (|t1| < |t0| / sqrt(2)): "twice is enough".
-double finish_gs(sbdg_ctx* c, Small& s, int m, int64_t N, const double* n0_dev) {
+// y32 set: the normalised vector is also written there in FP32 (the next FP32 sigma's input)
+double finish_gs(sbdg_ctx* c, Small& s, int m, int64_t N, const double* n0_dev, float* y32 = nullptr) {
   double* vm = c->V.p + (int64_t)m * N;
   double nt = 0.0, n0 = 0.0;
   gs_impl(c->V.p, N, m, s.dots, vm, N, s.norms + 8, s.part, c->st);
@@ -309,17 +310,19 @@ double finish_gs(sbdg_ctx* c, Small& s, int m, int64_t N, const double* n0_dev) 
   }
   if (!std::isfinite(nt)) throw Error(SBDG_ERR_NUMERIC, "non-finite correction vector");
   if (nt <= 0.0) return 0.0;
-  scale_impl(vm, s.norms + 8, N, c->st);
+  if (y32) scale_cast_impl(vm, s.norms + 8, y32, N, c->st);
+  else scale_impl(vm, s.norms + 8, N, c->st);
   return std::sqrt(nt);
 }
 
 // correction vector from the stored residual R: preconditioned, then Gram-Schmidt (the new
 // vector sits at V[m], so one more column of the dot pass gives |t|^2 before projection)
-double expand(sbdg_ctx* c, Small& s, int m, int64_t N, const double* theta_dev, const double* rn2_dev) {
+double expand(sbdg_ctx* c, Small& s, int m, int64_t N, const double* theta_dev, const double* rn2_dev,
+              float* y32 = nullptr) {
   double* vm = c->V.p + (int64_t)m * N;
   precond_impl(c->R.p, rn2_dev, c->d_hii.p, theta_dev, vm, N, c->st);
   multi_dot_impl(c->V.p, N, m + 1, vm, N, s.dots, s.part, c->st);
-  return finish_gs(c, s, m, N, s.dots + m);
+  return finish_gs(c, s, m, N, s.dots + m, y32);
 }
 
 void davidson(sbdg_ctx* c) {
@@ -439,8 +442,11 @@ void davidson(sbdg_ctx* c) {
     rnorm = std::sqrt(rr);
     w_valid = false;
   };
+  // mixed: c32 holds basis vector c32_of in FP32 when the normalisation wrote it there
+  int c32_of = -1;
   auto finish_fused = [&]() {
-    const double nt = finish_gs(c, s, v, N, s.dots + v + 1);
+    const double nt = finish_gs(c, s, v, N, s.dots + v + 1, mixed ? c->c32.p : nullptr);
+    c32_of = mixed ? v : -1;
     if (nt == 0.0) {
       stop = true;  // the residual lies in the span of the basis
       return;
@@ -478,7 +484,8 @@ void davidson(sbdg_ctx* c) {
   };
   auto expand_or_stop = [&]() {
     if (v >= nbmax) return;
-    const double nt = expand(c, s, v, N, th_exp, rr_exp);
+    const double nt = expand(c, s, v, N, th_exp, rr_exp, mixed ? c->c32.p : nullptr);
+    c32_of = mixed ? v : -1;
     if (nt == 0.0) {
       stop = true;  // the residual lies in the span of the basis
       return;
@@ -503,16 +510,19 @@ void davidson(sbdg_ctx* c) {
     // sigma of the newest basis vector and column h of the projected matrix
     if (mixed) {
       timed([&] {
-        cast_d2f(c->V.p + (int64_t)h * N, c->c32.p, N, c->st);
+        if (c32_of != h) cast_d2f(c->V.p + (int64_t)h * N, c->c32.p, N, c->st);
         sigma32(c, c->c32.p, c->HV32.p + (int64_t)h * N);
       });
+      c32_of = -1;
       c->stats.sigma_fp32++;
-      multi_dot_impl(c->V.p, N, h + 1, c->HV32.p + (int64_t)h * N, N, s.dots, s.part, c->st);
+      // column h of the projected matrix from the rows of the (symmetric) operator,
+      // <H V_k, V_h> with the FP32 sigma of every basis vector (V_h read once, the FP32
+      // vectors at half the bytes), and the exact <V_h, H x> for the anchor row
+      multi_dot_rows_impl(c->HV32.p, N, h + 1, c->V.p + (int64_t)h * N, have_anchor ? c->HX.p : nullptr, N,
+                          s.dots, s.part, c->st);
       SBDG_CHECK(hipMemcpyAsync(dots.data(), s.dots, sizeof(double) * (h + 1), hipMemcpyDeviceToHost, c->st));
-      if (have_anchor) {  // exact <V_h, H x> for the anchor row
-        multi_dot_impl(c->V.p + (int64_t)h * N, N, 1, c->HX.p, N, s.norms + 10, s.part, c->st);
-        SBDG_CHECK(hipMemcpyAsync(&dots[0], s.norms + 10, sizeof(double), hipMemcpyDeviceToHost, c->st));
-      }
+      if (have_anchor)
+        SBDG_CHECK(hipMemcpyAsync(&dots[0], s.dots + h + 1, sizeof(double), hipMemcpyDeviceToHost, c->st));
     } else {
       timed([&] { sigma64(c, c->V.p + (int64_t)h * N, c->HV.p + (int64_t)h * N); });
       c->stats.sigma_fp64++;

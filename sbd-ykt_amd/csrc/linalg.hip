@@ -144,6 +144,49 @@ __global__ void __launch_bounds__(RT) k_mdot(const double* __restrict__ X, int64
   }
 }
 
+// out partials of <Y_k, x> (k < m, Y FP32) and, with Z, of <z, x>: x is read once
+template <int MM, int VEC, bool Z>
+__global__ void __launch_bounds__(RT) k_mdot_rows(const float* __restrict__ Y, int64_t ldy, int m,
+                                                  const double* __restrict__ x, const double* __restrict__ z,
+                                                  int64_t n, double* __restrict__ part) {
+  __shared__ double sh[RT / 64];
+  double acc[MM], az = 0.0;
+#pragma unroll
+  for (int k = 0; k < MM; ++k) acc[k] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t u = (int64_t)blockIdx.x * RT + threadIdx.x; u < n / VEC; u += stride) {
+    const int64_t i = u * VEC;
+    double xv[VEC];
+    ld_vec<VEC>(x + i, xv);
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+      if (k < m) {
+        double yv[VEC];
+        ld_vec<VEC>(Y + k * ldy + i, yv);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[k] += yv[e] * xv[e];
+      }
+    if constexpr (Z) {
+      double zv[VEC];
+      ld_vec<VEC>(z + i, zv);
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) az += zv[e] * xv[e];
+    }
+  }
+  const int w = m + (Z ? 1 : 0);
+#pragma unroll
+  for (int k = 0; k < MM; ++k) {
+    if (k < m) {
+      const double v = block_sum(acc[k], sh);
+      if (threadIdx.x == 0) part[(int64_t)blockIdx.x * w + k] = v;
+    }
+  }
+  if constexpr (Z) {
+    const double v = block_sum(az, sh);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * w + m] = v;
+  }
+}
+
 template <class TH, int MM, int VEC>
 __global__ void __launch_bounds__(RT) k_ritz(const double* __restrict__ V, int64_t ldv,
                                              const TH* __restrict__ HV, int64_t ldh, int m,
@@ -263,6 +306,20 @@ __global__ void __launch_bounds__(RT) k_ritz_expand(const double* __restrict__ V
   if (threadIdx.x == 0) {
     part[(int64_t)blockIdx.x * (m + 2) + m] = sr;
     part[(int64_t)blockIdx.x * (m + 2) + m + 1] = st;
+  }
+}
+
+template <int VEC>
+__global__ void k_scale_cast(double* __restrict__ x, const double* __restrict__ n2, float* __restrict__ y,
+                             int64_t n) {
+  const double f = 1.0 / sqrt(n2[0]);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n / VEC; q += (int64_t)gridDim.x * blockDim.x) {
+    double v[VEC];
+    ld_vec<VEC>(x + q * VEC, v);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) v[e] *= f;
+    st_vec<VEC>(x + q * VEC, v);
+    st_vec<VEC>(y + q * VEC, v);
   }
 }
 
@@ -569,6 +626,30 @@ template void ritz_expand_impl<double>(const double*, int64_t, const double*, in
 template void ritz_expand_impl<float>(const double*, int64_t, const float*, int64_t, int, const double*,
                                       const double*, const double*, double*, int64_t, double*, double*,
                                       hipStream_t);
+
+void scale_cast_impl(double* x, const double* n2, float* y, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  if (pairs(n, 0)) hipLaunchKernelGGL(k_scale_cast<2>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, y, n);
+  else hipLaunchKernelGGL(k_scale_cast<1>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, y, n);
+}
+
+void multi_dot_rows_impl(const float* Y, int64_t ldy, int m, const double* x, const double* z, int64_t n,
+                         double* out, double* part, hipStream_t st) {
+  const unsigned g = nblocks(n);
+  dispatch_m<0>(m, [&](auto MM) {
+    constexpr int mm = decltype(MM)::value;
+    const bool v2 = pairs(n, ldy);
+    if (z) {
+      if (v2) hipLaunchKernelGGL((k_mdot_rows<mm, 2, true>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+      else hipLaunchKernelGGL((k_mdot_rows<mm, 1, true>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+    } else {
+      if (v2) hipLaunchKernelGGL((k_mdot_rows<mm, 2, false>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+      else hipLaunchKernelGGL((k_mdot_rows<mm, 1, false>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+    }
+  });
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RT), 0, st, part, (int)g, m + (z ? 1 : 0), out);
+  SBDG_CHECK(hipGetLastError());
+}
 
 void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st) {
   if (n == 0) return;

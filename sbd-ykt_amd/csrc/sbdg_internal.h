@@ -204,6 +204,8 @@ void cast_f2d(const float* in, double* out, int64_t n, hipStream_t st);
 // out[k] = <X_k, y> for k < m (deterministic two-stage reduction); y FP64 or FP32.
 void multi_dot_impl(const double* X, int64_t ldx, int m, const double* y, int64_t n, double* out, double* part,
                     hipStream_t st);
+void multi_dot_rows_impl(const float* Y, int64_t ldy, int m, const double* x, const double* z, int64_t n,
+                         double* out, double* part, hipStream_t st);
 void multi_dot_impl(const double* X, int64_t ldx, int m, const float* y, int64_t n, double* out, double* part,
                     hipStream_t st);
 // W = sum_k u[k] V_k ; R = sum_k u[k] HV_k - theta W ; out[0] = |W|^2, out[1] = |R|^2
@@ -217,6 +219,10 @@ void ritz_expand_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, i
                       const double* theta, const double* hii, double* T, int64_t n, double* out, double* part,
                       hipStream_t st);
 void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st);  // x /= sqrt(n2[0])
+// x /= sqrt(n2[0]) and y = (float)x in the same pass
+void scale_cast_impl(double* x, const double* n2, float* y, int64_t n, hipStream_t st);
+// out[k] = <Y_k, x> for k < m (Y FP32, Y_k = Y + k*ldy), and out[m] = <z, x> when z is set:
+// one pass that reads x once
 // t = (R/|R|) / (theta - d), with the 1e-12 regularisation of davidson.h:855-862
 void precond_impl(const double* R, const double* rn2, const double* hii, const double* theta, double* t, int64_t n,
                   hipStream_t st);
