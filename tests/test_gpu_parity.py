@@ -140,6 +140,28 @@ def test_sigma_columns_are_matrix_columns(doubles):
         assert np.abs(col32 - H[:, J]).max() < 2e-5 * np.abs(H[:, J]).max()
 
 
+@pytest.mark.parametrize("L,na_e,nb_e,na,nb", [
+    (6, 3, 2, 1, 15),  # one alpha string: no alpha excitation at all
+    (6, 2, 3, 15, 1),  # one beta string: a single column, no beta excitation
+    (5, 0, 2, 0, 0),   # no alpha electron (the empty string), beta full space
+    (5, 2, 0, 0, 0),   # no beta electron
+    (6, 1, 1, 0, 0),   # one electron per spin: no same-spin double anywhere
+    (7, 7, 1, 0, 0),   # a full alpha shell: one alpha string holding every orbital
+])
+def test_edge_string_sets_vs_bruteforce(L, na_e, nb_e, na, nb):
+    """Degenerate string sets: sigma (FP64 and FP32) and the ground-state energy against the
+    dense brute-force Hamiltonian."""
+    case = workload_small(L, na_e, nb_e, na, nb, 5)
+    H = bf.hamiltonian(case["norb"], case["core"], case["h1"], case["eri"], case["astr"], case["bstr"])
+    s = solver(case, max_it=100, max_nb=8, eps=1e-10)
+    x = np.random.default_rng(9).normal(size=s.ndet)
+    hx = H @ x
+    assert np.abs(s.sigma(x) - hx).max() <= 1e-12 * max(1.0, np.abs(hx).max())
+    assert np.abs(s.sigma(x, precision=1) - hx).max() <= 2e-5 * max(1.0, np.abs(hx).max())
+    e = s.run()
+    assert abs(e - np.linalg.eigvalsh(H)[0]) < 1e-8
+
+
 @pytest.mark.parametrize("doubles", [1, 2])
 def test_sigma_medium_vs_oracle_and_symmetry(doubles):
     case = medium_case()
