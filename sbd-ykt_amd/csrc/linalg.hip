@@ -37,11 +37,26 @@ __device__ inline double block_sum(double v, double* sh) {
   return t;
 }
 
-// VEC consecutive elements at p as doubles (VEC == 2: p is 2-element aligned, one
-// 16-byte load for FP64, 8-byte for FP32), and the store back
+// VEC consecutive elements at p as doubles (VEC == 2 or 4: p is VEC-element aligned;
+// 16-byte loads for FP64, 8- or 16-byte for FP32), and the store back
 template <int VEC, class T>
 __device__ inline void ld_vec(const T* __restrict__ p, double (&o)[VEC]) {
-  if constexpr (VEC == 2) {
+  if constexpr (VEC == 4) {
+    if constexpr (sizeof(T) == 8) {
+      const double2 a = *reinterpret_cast<const double2*>(p);
+      const double2 b = *reinterpret_cast<const double2*>(p + 2);
+      o[0] = a.x;
+      o[1] = a.y;
+      o[2] = b.x;
+      o[3] = b.y;
+    } else {
+      const float4 v = *reinterpret_cast<const float4*>(p);
+      o[0] = (double)v.x;
+      o[1] = (double)v.y;
+      o[2] = (double)v.z;
+      o[3] = (double)v.w;
+    }
+  } else if constexpr (VEC == 2) {
     if constexpr (sizeof(T) == 8) {
       const double2 v = *reinterpret_cast<const double2*>(p);
       o[0] = v.x;
@@ -57,7 +72,14 @@ __device__ inline void ld_vec(const T* __restrict__ p, double (&o)[VEC]) {
 }
 template <int VEC, class T>
 __device__ inline void st_vec(T* __restrict__ p, const double (&v)[VEC]) {
-  if constexpr (VEC == 2) {
+  if constexpr (VEC == 4) {
+    if constexpr (sizeof(T) == 8) {
+      *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+      *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+    } else {
+      *reinterpret_cast<float4*>(p) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+    }
+  } else if constexpr (VEC == 2) {
     if constexpr (sizeof(T) == 8) *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
     else *reinterpret_cast<float2*>(p) = make_float2((float)v[0], (float)v[1]);
   } else {
@@ -526,8 +548,10 @@ void dispatch_m(int m, F&& f) {
   else throw Error(SBDG_ERR_LIMIT, "more than 64 subspace vectors");
 }
 
-// two elements per thread and 16-byte loads when every vector starts 2-element aligned
+// two or four consecutive elements per thread (16-byte loads, longer runs per stream)
+// when every vector starts 2- or 4-element aligned
 inline bool pairs(int64_t n, int64_t ld1, int64_t ld2 = 0) { return n % 2 == 0 && ld1 % 2 == 0 && ld2 % 2 == 0; }
+inline bool quads(int64_t n, int64_t ld1, int64_t ld2 = 0) { return n % 4 == 0 && ld1 % 4 == 0 && ld2 % 4 == 0; }
 
 }  // namespace
 
@@ -558,7 +582,9 @@ void multi_dot_impl(const double* X, int64_t ldx, int m, const double* y, int64_
                     hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    if (pairs(n, ldx))
+    if (quads(n, ldx))
+      hipLaunchKernelGGL((k_mdot<double, decltype(MM)::value, 4>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+    else if (pairs(n, ldx))
       hipLaunchKernelGGL((k_mdot<double, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
     else
       hipLaunchKernelGGL((k_mdot<double, decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
@@ -571,7 +597,9 @@ void multi_dot_impl(const double* X, int64_t ldx, int m, const float* y, int64_t
                     hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    if (pairs(n, ldx))
+    if (quads(n, ldx))
+      hipLaunchKernelGGL((k_mdot<float, decltype(MM)::value, 4>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
+    else if (pairs(n, ldx))
       hipLaunchKernelGGL((k_mdot<float, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
     else
       hipLaunchKernelGGL((k_mdot<float, decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, X, ldx, m, y, n, part);
@@ -586,7 +614,10 @@ void ritz_combine_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, 
                        hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    if (pairs(n, ldv, ldh))
+    if (quads(n, ldv, ldh))
+      hipLaunchKernelGGL((k_ritz<TH, decltype(MM)::value, 4>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u,
+                         theta, W, R, n, part);
+    else if (pairs(n, ldv, ldh))
       hipLaunchKernelGGL((k_ritz<TH, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh, m, u,
                          theta, W, R, n, part);
     else
@@ -609,7 +640,10 @@ void ritz_expand_impl(const double* V, int64_t ldv, const TH* HV, int64_t ldh, i
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
     if constexpr (decltype(MM)::value <= 32) {
-      if (pairs(n, ldv, ldh))
+      if (decltype(MM)::value <= 16 && quads(n, ldv, ldh))
+        hipLaunchKernelGGL((k_ritz_expand<TH, decltype(MM)::value, (decltype(MM)::value <= 16 ? 4 : 2)>), dim3(g),
+                           dim3(RT), 0, st, V, ldv, HV, ldh, m, u, theta, hii, T, n, part);
+      else if (pairs(n, ldv, ldh))
         hipLaunchKernelGGL((k_ritz_expand<TH, decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, V, ldv, HV, ldh,
                            m, u, theta, hii, T, n, part);
       else
@@ -629,7 +663,8 @@ template void ritz_expand_impl<float>(const double*, int64_t, const float*, int6
 
 void scale_cast_impl(double* x, const double* n2, float* y, int64_t n, hipStream_t st) {
   if (n == 0) return;
-  if (pairs(n, 0)) hipLaunchKernelGGL(k_scale_cast<2>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, y, n);
+  if (quads(n, 0)) hipLaunchKernelGGL(k_scale_cast<4>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, y, n);
+  else if (pairs(n, 0)) hipLaunchKernelGGL(k_scale_cast<2>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, y, n);
   else hipLaunchKernelGGL(k_scale_cast<1>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, y, n);
 }
 
@@ -638,12 +673,14 @@ void multi_dot_rows_impl(const float* Y, int64_t ldy, int m, const double* x, co
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
     constexpr int mm = decltype(MM)::value;
-    const bool v2 = pairs(n, ldy);
+    const bool v2 = pairs(n, ldy), v4 = quads(n, ldy);
     if (z) {
-      if (v2) hipLaunchKernelGGL((k_mdot_rows<mm, 2, true>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+      if (v4) hipLaunchKernelGGL((k_mdot_rows<mm, 4, true>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+      else if (v2) hipLaunchKernelGGL((k_mdot_rows<mm, 2, true>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
       else hipLaunchKernelGGL((k_mdot_rows<mm, 1, true>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
     } else {
-      if (v2) hipLaunchKernelGGL((k_mdot_rows<mm, 2, false>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+      if (v4) hipLaunchKernelGGL((k_mdot_rows<mm, 4, false>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
+      else if (v2) hipLaunchKernelGGL((k_mdot_rows<mm, 2, false>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
       else hipLaunchKernelGGL((k_mdot_rows<mm, 1, false>), dim3(g), dim3(RT), 0, st, Y, ldy, m, x, z, n, part);
     }
   });
@@ -653,7 +690,8 @@ void multi_dot_rows_impl(const float* Y, int64_t ldy, int m, const double* x, co
 
 void scale_impl(double* x, const double* n2, int64_t n, hipStream_t st) {
   if (n == 0) return;
-  if (pairs(n, 0)) hipLaunchKernelGGL(k_scale<2>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
+  if (quads(n, 0)) hipLaunchKernelGGL(k_scale<4>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
+  else if (pairs(n, 0)) hipLaunchKernelGGL(k_scale<2>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
   else hipLaunchKernelGGL(k_scale<1>, dim3(nblocks(n) * 4), dim3(256), 0, st, x, n2, n);
 }
 
@@ -667,7 +705,9 @@ void gs_impl(const double* V, int64_t ldv, int m, const double* s, double* t, in
              hipStream_t st) {
   const unsigned g = nblocks(n);
   dispatch_m<0>(m, [&](auto MM) {
-    if (pairs(n, ldv))
+    if (quads(n, ldv))
+      hipLaunchKernelGGL((k_gs<decltype(MM)::value, 4>), dim3(g), dim3(RT), 0, st, V, ldv, m, s, t, n, part);
+    else if (pairs(n, ldv))
       hipLaunchKernelGGL((k_gs<decltype(MM)::value, 2>), dim3(g), dim3(RT), 0, st, V, ldv, m, s, t, n, part);
     else
       hipLaunchKernelGGL((k_gs<decltype(MM)::value, 1>), dim3(g), dim3(RT), 0, st, V, ldv, m, s, t, n, part);
