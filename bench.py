@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16)
     ap.add_argument("--sequential", action="store_true", help="time sigma with its kernels in one stream")
     ap.add_argument("--tts-starts", type=int, default=1, help="start vectors of the time-to-solution runs (1-3)")
+    ap.add_argument("--davidson-steps", action="store_true",
+                    help="only run 20 FP64 and 24 mixed Davidson steps (for a kernel trace of the vector passes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (scales the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -166,6 +168,15 @@ def main():
     na, nb = len(wl.astr), len(wl.bstr)
     nnz = st["nnz"]
     print(f"[bench] tables+diagonal {st['t_tables']:.2f} s, nnz {nnz}, terms/sigma {st['terms']:.3e}", flush=True)
+    if args.davidson_steps:
+        for prec in (0, 1):
+            s.set_options(max_it=2, max_nb=12, restart_keep=4, eps=1e-12, precision=prec, refresh_k=0)
+            s.run()
+            sd = s.stats()
+            print(f"[bench] davidson precision {prec}: {sd['iterations']} steps in {sd['t_davidson']:.3f} s, "
+                  f"sigma {sd['t_sigma']:.3f} s", flush=True)
+        s.close()
+        return
 
     # warmup, then K timed FP64 sigma builds (library events per kernel; wall clock overall)
     s.bench_sigma(0, max(1, args.warmup))
