@@ -3,8 +3,9 @@
 // The driver follows Davidson(...,max_time) (tpb/davidson.h:661-924): lowest root,
 // diagonal preconditioner on the normalised residual with the 1e-12 regularisation,
 // Gram-Schmidt against the basis, convergence on |HW - E W| < eps, the max_time guard.
-// Differences, all numerically neutral: Gram-Schmidt is classical and applied twice
-// (each pass one multi-dot and one update) instead of one modified pass; at a restart
+// Differences, all numerically neutral: Gram-Schmidt is classical (each pass one
+// multi-dot and one update; a second pass only when the first removed more than half of
+// the vector) instead of one modified pass; at a restart
 // the kept Ritz vectors carry their sigma (HV rotated with the same coefficients), so
 // the sigma of the restarted vector is not recomputed; restart_keep > 1 keeps that many
 // Ritz vectors (thick restart).  Every vector stays in HBM; the host sees only the
