@@ -88,7 +88,7 @@ def ab_lds_bytes(na, nb, nnz, nel_b, s):
     return sa * sb * 2 * s
 
 
-def survey_b_alg(na, nb, nnz, s):
+def row_stream_alg_bytes(na, nb, nnz, s):
     """Per-sigma algorithmic bytes of the row-streamed model:
     N 3s + s Nb (nnzSa + nnzDa) + s Nb nnzSa + 4 nnz(tables)."""
     sa, da, sb, db = nnz
@@ -210,7 +210,7 @@ def main():
         lb = ab_lds_bytes(na, nb, nnz, nel_b, 8)
         roofline["lds"] = dict(bytes=int(lb), achieved_TBs=round(lb / (per[dom] * 1e-3) / 1e12, 2),
                                peak_TBs=LDS_PEAK_TBS, frac=round(lb / (per[dom] * 1e-3) / 1e12 / LDS_PEAK_TBS, 4))
-    b_alg = survey_b_alg(na, nb, nnz, 8)
+    b_alg = row_stream_alg_bytes(na, nb, nnz, 8)
     sigma_model = dict(alg_bytes=int(b_alg), achieved_GBs=round(b_alg / (ms_step * 1e-3) / 1e9, 1),
                        frac=round(b_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        sequential_ms=round(float(ms[0]), 3), kernel_ms={k: round(v, 3) for k, v in per.items()})
