@@ -202,6 +202,40 @@ def test_sigma_c1_shape_row_sample(doubles, core_gib):
         assert np.abs(sx32[blk] - sx[blk]).max() <= 2e-5 * np.abs(sx[blk]).max()
 
 
+def test_sigma_c2_full_size_rows_and_symmetry():
+    """Full c2 shape (norb=36, 9+9 electrons, 20,000 x 20,000 strings, 400M determinants),
+    the benchmark's workload: sigma on the first and the last alpha row equals the oracle
+    restricted to those rows; <y, H x> = <H y, x> over the whole space; the alpha-beta
+    kernel in sequence gives the same sigma as beside the spin phases (summation order
+    only); FP32 sigma on the sampled rows within 2e-5."""
+    wl = syn.config_workload("c2_L36_9e9e_20000")
+    case = dict(norb=wl.norb, core=wl.core, h1=wl.h1, eri=wl.eri, astr=wl.astr, bstr=wl.bstr)
+    s, p = solver(case), oproblem(case)
+    nb = s.nb
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=s.ndet)
+    sx = s.sigma(x)
+    for ia in (0, s.na - 1):
+        o = p.sigma_rows(x, None, ia, ia + 1)
+        blk = slice(ia * nb, (ia + 1) * nb)
+        assert np.abs(sx[blk] - o[blk]).max() <= 1e-12 * np.abs(o[blk]).max()
+        del o
+    sx32 = s.sigma(x, precision=1)
+    for ia in (0, s.na - 1):
+        blk = slice(ia * nb, (ia + 1) * nb)
+        assert np.abs(sx32[blk] - sx[blk]).max() <= 2e-5 * np.abs(sx[blk]).max()
+    del sx32
+    s.set_options(overlap=0)
+    sq = s.sigma(x)
+    assert np.abs(sq - sx).max() <= 1e-13 * np.abs(sx).max()
+    del sq
+    y = rng.normal(size=s.ndet)
+    sy = s.sigma(y)
+    a, b = float(np.dot(y, sx)), float(np.dot(sy, x))
+    assert abs(a - b) <= 1e-12 * np.sqrt(float(np.dot(sx, sx)) * float(np.dot(y, y)))
+    s.close()
+
+
 @pytest.mark.parametrize("overlap", [1, 0])
 def test_sigma_wide_beta_set_vs_oracle(overlap):
     """30,000 beta strings (norb=24, 5 electrons) against 12 alpha strings: the alpha-beta
