@@ -108,10 +108,10 @@ double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// column chunk of the alpha-beta kernel: ket-row chunks and L*L slices of 16-byte row
-// vectors (2 FP64 or 4 FP32 rows) in LDS
+// column chunk of the alpha-beta kernel: ket-row chunks and the integral slices of 16-byte
+// row vectors (2 FP64 or 4 FP32 rows) in LDS
 int64_t choose_chunk(int64_t nb, int L) {
-  const int64_t budget = (int64_t)(LDS_BUDGET / 16) - (int64_t)L * L - 1;
+  const int64_t budget = (int64_t)(LDS_BUDGET / 16) - ab_slice_span(L) - 1;
   int64_t ch = std::max<int64_t>(256, budget);
   return std::min<int64_t>(std::max<int64_t>(nb, 1), ch);
 }

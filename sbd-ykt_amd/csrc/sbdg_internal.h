@@ -107,14 +107,16 @@ struct AbTable {
 // multiple of four rows of 64 lane-interleaved entries (nq[stream*m + j] batches of four
 // rows) padded with a zero-weight entry: the wave reads its stream coalesced, four rows
 // at a time, with the next four rows in flight across group boundaries.
-// Entry: bits 0-17 LDS byte address of the ket vector (16 * (L*L + column offset inside
-// the chunk); offset `chunk` is a zero vector: padding), bits 18-29 orbital pair b*L+j
-// (its slice vector sits at byte 16 * (b*L+j)), bit 31 sign.  Within a lane the entries
-// are ordered so that lanes read distinct LDS slots where possible.
+// Entry: bits 0-17 LDS byte address of the ket vector (16 * (kbase + column offset inside
+// the chunk); offset `chunk` is a zero vector: padding), bits 18-29 the slice vector of
+// orbital pair b*L+j (b*L+j, or sb2 + b*L+j in the second copy of the slices), bit 31
+// sign.  Within a lane the entries are ordered, and each takes the slice copy, so that
+// lanes read distinct LDS slots where possible.
 struct EllTable {
   int nch = 0;
   int nt = 0, m = 0;        // alpha-beta launch: threads per workgroup, columns per thread
   int64_t chunk = 0, groups = 0, ntiles = 0;
+  int64_t sb2 = 0, kbase = 0;  // second slice copy (0: one copy), first ket vector (16-byte units)
   DBuf<int64_t> soff;       // ntiles*(nt/64)*nch + 1: start of each wave's entry stream per chunk
   DBuf<int32_t> nq;         // ntiles*(nt/64)*nch*m: four-row batches of each group of the stream
   DBuf<uint32_t> ent;
@@ -192,7 +194,21 @@ struct SigmaPlan {
 template <class T>
 void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream_t st,
                  hipEvent_t* ev = nullptr);
-size_t ab_lds_bytes(int L, int64_t chunk);
+size_t ab_lds_bytes(int64_t kbase, int64_t chunk);
+
+// The alpha-beta kernel's LDS in 16-byte vectors: L*L integral slice vectors, a second copy
+// of them starting 8 slots (128 bytes) further along the 256-byte bank row when an entry's
+// 12-bit slice field reaches it (the entry schedule then picks, per entry, the copy whose
+// bank is free), then the ket vectors of one chunk.
+inline int64_t ab_second_slice(int L) {
+  const int64_t LL = (int64_t)L * L;
+  const int64_t b = LL + (24 - LL % 16) % 16;  // b % 16 == 8
+  return b + LL <= 4096 ? b : 0;
+}
+inline int64_t ab_slice_span(int L) {
+  const int64_t b = ab_second_slice(L);
+  return b ? b + (int64_t)L * L : (int64_t)L * L;
+}
 
 // vector kernels (linalg.hip)
 void sign_transform(const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, const double* in,

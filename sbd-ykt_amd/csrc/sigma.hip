@@ -303,7 +303,8 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
 // ja in S(ia) are taken G at a time (G = 2 in FP64, 4 in FP32: 16 bytes either way):
 // the G (ai|..) slices and the G rows' current column chunk sit in LDS interleaved, so one
 // 16-byte LDS read serves all G rows, and every ELL entry read from the L2-resident table
-// serves them too.  LDS: [L*L slice G-vectors] [chunk + 1 ket G-vectors, the last one zero].
+// serves them too.  LDS: [L*L slice G-vectors] ([a second copy at sb2]) [chunk + 1 ket
+// G-vectors from kbase, the last one zero].
 template <class T>
 struct AbRows {
   static constexpr int G = 16 / (int)sizeof(T);
@@ -319,12 +320,18 @@ struct AbRows {
   }
 };
 
-// the (ai|..) slice of one row into LDS: element x of row g lives at V[G*x + g]
+// the (ai|..) slice of one row into LDS: element x of row g lives at V[G*x + g] and, with a
+// second copy (sb2 > 0), at V[G*(sb2 + x) + g]; ai < 0: a zero row
 template <class T>
-__device__ inline void ab_slice(int ai, int64_t LL, const T* __restrict__ eri, T* __restrict__ V, int g) {
+__device__ inline void ab_slice(int ai, int64_t LL, int64_t sb2, const T* __restrict__ eri, T* __restrict__ V,
+                                int g) {
   constexpr int G = AbRows<T>::G;
-  const T* src = eri + (int64_t)ai * LL;
-  for (int x = threadIdx.x; x < LL; x += blockDim.x) V[G * x + g] = src[x];
+  const T* src = eri + (int64_t)(ai < 0 ? 0 : ai) * LL;
+  for (int x = threadIdx.x; x < LL; x += blockDim.x) {
+    const T v = ai < 0 ? T(0) : src[x];
+    V[G * x + g] = v;
+    if (sb2) V[G * (sb2 + x) + g] = v;
+  }
 }
 
 // The LDS vector at byte address a of the workgroup's LDS.  The alpha-beta kernel has no
@@ -379,14 +386,15 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
                                              const int64_t* __restrict__ soff, const int32_t* __restrict__ nq,
                                              const uint32_t* __restrict__ ent,
                                              const int32_t* __restrict__ colperm, int nch, int64_t chunk,
+                                             int64_t sb2, int64_t kbase,
                                              int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
                                              T* __restrict__ sigma, int assign) {
   using R = AbRows<T>;
   using RV = typename R::V;
   constexpr int G = R::G;
   extern __shared__ __align__(16) unsigned char smem[];
-  RV* Vv = reinterpret_cast<RV*>(smem);  // L*L G-vectors
-  RV* cbv = Vv + (int64_t)L * L;          // chunk + 1 G-vectors (the last one stays zero: padding)
+  RV* Vv = reinterpret_cast<RV*>(smem);  // L*L slice G-vectors (twice when sb2 > 0)
+  RV* cbv = Vv + kbase;                   // chunk + 1 G-vectors (the last one stays zero: padding)
   T* Vs = reinterpret_cast<T*>(Vv);
   const int64_t LL = (int64_t)L * L;
   const int NT = blockDim.x;
@@ -435,9 +443,9 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
           const uint32_t e = excSa[q];
           coef[g] = ((e >> 24) & 1u) ? T(-1) : T(1);
           crow[g] = c + (int64_t)colSa[q] * nb + j0;
-          ab_slice<T>((int)((e >> 6) & 63u) * L + (int)(e & 63u), LL, eri, Vs, g);  // a*L + i
+          ab_slice<T>((int)((e >> 6) & 63u) * L + (int)(e & 63u), LL, sb2, eri, Vs, g);  // a*L + i
         } else {
-          for (int x = threadIdx.x; x < LL; x += NT) Vs[G * x + g] = T(0);
+          ab_slice<T>(-1, LL, sb2, eri, Vs, g);
         }
       }
       // the thread index, opaque here so that the fill's addresses are formed per step
@@ -607,7 +615,7 @@ void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* si
   if (!lds_at_zero) throw Error(SBDG_ERR_STATE, "alpha-beta kernel has static LDS: entry addresses invalid");
   hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.nb, P.L, E.ntiles,
                      P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.soff.p, E.nq.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
-                     E.groups, eri, c, sigma, assign);
+                     E.sb2, E.kbase, E.groups, eri, c, sigma, assign);
 }
 
 template <class T>
@@ -615,7 +623,7 @@ void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream
   if (P.na == 0 || P.nb == 0) return;
   const EllTable& E = *P.ell;
   if ((int64_t)E.nt * E.m * E.ntiles != E.groups * 64) throw Error(SBDG_ERR_STATE, "alpha-beta table geometry");
-  const size_t lds = ab_lds_bytes(P.L, E.chunk);
+  const size_t lds = ab_lds_bytes(E.kbase, E.chunk);
   switch (E.m) {  // columns per thread, fixed when the table was built
     case 1: launch_ab_m<T, 1>(P, lds, eri, c, sigma, st, assign); break;
     case 2: launch_ab_m<T, 2>(P, lds, eri, c, sigma, st, assign); break;
@@ -636,8 +644,8 @@ void transpose_d(int64_t R, int64_t C, const double* in, double* out, hipStream_
   SBDG_CHECK(hipGetLastError());
 }
 
-size_t ab_lds_bytes(int L, int64_t chunk) {
-  return (size_t)16 * ((size_t)chunk + 1 + (size_t)L * L);  // 16-byte row vectors, any precision
+size_t ab_lds_bytes(int64_t kbase, int64_t chunk) {
+  return (size_t)16 * ((size_t)chunk + 1 + (size_t)kbase);  // 16-byte row vectors, any precision
 }
 
 template <class T>

@@ -494,12 +494,13 @@ static void ab_geometry(int64_t nb, int& nt, int& m) {
 // element size) in both the ket-row array and the integral array; equal addresses
 // broadcast and cost nothing.  Padding (the zero slot, pair 0) may be placed at any step
 // while a lane has slack.  Greedy: at each step the lanes with least slack choose first,
-// each taking the entry that adds the fewest new addresses to slots already in use.
-// lists[l] holds lane l's entries (ket byte address | pair << 18 | sign << 31); out gets
-// K*64.  Slots count 16-byte LDS vectors: the pair's slice vector is the pair index, the
+// each taking the entry, and with a second slice copy (sb2 > 0) the copy, that adds the
+// fewest new addresses to slots already in use.  lists[l] holds lane l's entries (ket
+// byte address | pair << 18 | sign << 31); out gets K*64.  Slots count 16-byte LDS
+// vectors: the pair's slice vector is the pair index (sb2 + pair in the second copy), the
 // ket's is its byte address / 16.
 static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int P, int S, uint32_t pad,
-                           uint32_t* out) {
+                           uint32_t sb2, uint32_t* out) {
   struct Use {
     uint32_t addr[8];
     int n;
@@ -526,13 +527,22 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
         const int slack = (K - e) - rem[l];
         auto& ls = lists[l];
         int best = -1, bc = 1 << 30;
+        uint32_t bv = 0;  // slice vector of the best entry
         for (int j = 0; j < rem[l]; ++j) {
           const uint32_t w = ls[j];
-          const uint32_t ca = (w & 0x3FFFFu) >> 4, va = (w >> 18) & 0xFFFu;
-          const int cst = cost1(uc[ca % S], ca) + cost1(uv[va % S], va);
+          const uint32_t ca = (w & 0x3FFFFu) >> 4;
+          uint32_t va = (w >> 18) & 0xFFFu;
+          int cv = cost1(uv[va % S], va);
+          if (sb2 && cv > 0) {
+            const uint32_t vb = va + sb2;
+            const int c2 = cost1(uv[vb % S], vb);
+            if (c2 < cv) cv = c2, va = vb;
+          }
+          const int cst = cost1(uc[ca % S], ca) + cv;
           if (cst < bc) {
             bc = cst;
             best = j;
+            bv = va;
             if (cst == 0) break;
           }
         }
@@ -543,7 +553,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
           if (best < 0 || cst < bc) best = -1;
         }
         if (best >= 0) {
-          w = ls[best];
+          w = (ls[best] & ~(0xFFFu << 18)) | (bv << 18);
           ls[best] = ls[rem[l] - 1];  // order of the rest does not matter
           --rem[l];
         }
@@ -558,8 +568,10 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
 
 void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st) {
   const int64_t nb = sb.n;
-  if (16 * ((int64_t)L * L + chunk + 1) > (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta LDS beyond 256 KiB");
   if (L > 64) throw Error(SBDG_ERR_LIMIT, "norb beyond 64");
+  e.sb2 = ab_second_slice(L);
+  e.kbase = ab_slice_span(L);
+  if (16 * (e.kbase + chunk + 1) > (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta LDS beyond 256 KiB");
   e.chunk = chunk;
   e.nch = (int)std::max<int64_t>(1, (nb + chunk - 1) / chunk);
   ab_geometry(nb, e.nt, e.m);
@@ -650,13 +662,13 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
         soff[s + 1] = o;
       }
   const int64_t total = soff[nstream];
-  const uint32_t pad = ((uint32_t)(L * L) + (uint32_t)chunk) << 4;  // zero slot, pair 0
+  const uint32_t pad = ((uint32_t)e.kbase + (uint32_t)chunk) << 4;  // zero slot, pair 0
   std::vector<uint32_t> ent((size_t)std::max<int64_t>(total, 1), pad);
   // pass groups of the kernel's 16-byte LDS reads, as measured on gfx950 (the alpha-beta
   // kernel at norb=36, 20,000^2 with four candidate models; this one was fastest): 32 lanes
   // over 16 slots of 16 bytes (256 bytes of banks)
   const int P = elem >= 16 ? 32 : 16, S = 16;
-  const uint32_t kbase = (uint32_t)(L * L);  // ket vectors follow the L*L slice vectors
+  const uint32_t kbase = (uint32_t)e.kbase;  // ket vectors follow the slice vectors
   std::vector<std::vector<uint32_t>> lists(64);
   for (int64_t g = 0; g < e.groups; ++g)
     for (int k = 0; k < nch; ++k) {
@@ -674,7 +686,7 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
           lst.push_back(((kbase + (uint32_t)(col[q] - j0)) << 4) | ((b * (uint32_t)L + j) << 18) | (neg << 31));
         }
       }
-      schedule_block(lists, K[g * nch + k], P, S, pad, ent.data() + base);
+      schedule_block(lists, K[g * nch + k], P, S, pad, (uint32_t)e.sb2, ent.data() + base);
     }
   e.colperm.alloc(perm.size());
   SBDG_CHECK(hipMemcpy(e.colperm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
