@@ -49,14 +49,16 @@ def measured_traffic(kernel):
     return None if v is None else int(v)
 
 
-def kernel_bytes(na, nb, nnz, core, nel_b, s):
+def kernel_bytes(na, nb, nnz, core, nel_b, s, direct=(0, 0)):
     """Algorithmic HBM bytes per launch of each sigma phase (s = bytes per element).
 
     transpose:     read c, write c^T                                   2 N s
     beta product:  on c^T: singles rows and their occupation-table
                    rows, diagonal-free write                           2 s na nnzSb + N s + 16 nnzSb
-                   doubles by core (mb memberships): member rows in,
-                   partial rows out, partial rows gathered              + 3 s na mb
+                   doubles by core (mb memberships of the larger
+                   cores): member rows in, partial rows out, partial
+                   rows gathered; the xb pairs of the smaller cores
+                   read directly                                       + 3 s na mb + s na xb + 16 xb
                    (excitation-table form instead: s na nnzDb + 12 nnzDb)
     alpha product: same on c, with hii and c read for the diagonal     2 s nb nnzSa + 3 N s + ...
     transpose-add: read sigma^T and sigma, write sigma                 3 N s
@@ -66,16 +68,17 @@ def kernel_bytes(na, nb, nnz, core, nel_b, s):
     N = na * nb
     sa, da, sb, db = nnz
     ma, mb = core
+    xa, xb = direct
 
-    def spin(ncols, singles, doubles, members):
-        if members:
-            return s * ncols * (2 * singles + 3 * members) + 16 * singles
+    def spin(ncols, singles, doubles, members, pairs):
+        if members or pairs:
+            return s * ncols * (2 * singles + 3 * members + pairs) + 16 * (singles + pairs)
         return s * ncols * (2 * singles + doubles) + 16 * (singles + doubles)
 
     return {
         "transpose": 2 * N * s,
-        "beta": spin(na, sb, db, mb) + N * s,
-        "alpha": spin(nb, sa, da, ma) + 3 * N * s,
+        "beta": spin(na, sb, db, mb, xb) + N * s,
+        "alpha": spin(nb, sa, da, ma, xa) + 3 * N * s,
         "transpose_add": 3 * N * s,
         "alpha_beta": s * nb * sa + 2 * N * s + 4 * sb,
     }
@@ -199,7 +202,7 @@ def main():
     names = ["transpose", "beta", "alpha", "transpose_add", "alpha_beta"]
     nel_b = bin(int(wl.bstr[0])).count("1")
     core = st["core_members"]
-    kb = kernel_bytes(na, nb, nnz, core, nel_b, 8)
+    kb = kernel_bytes(na, nb, nnz, core, nel_b, 8, st["core_direct"])
     per = {n: float(ms[i + 1]) for i, n in enumerate(names)}
     dom = max(per, key=per.get)
     achieved = kb[dom] / (per[dom] * 1e-3) / 1e9

@@ -84,6 +84,8 @@ typedef struct {
   int64_t nnz[4];       /* alpha singles, alpha doubles, beta singles, beta doubles */
   int64_t core_members[2]; /* alpha, beta: memberships of the core-grouped doubles (0: the
                               excitation-table form is used for that spin) */
+  int64_t core_direct[2];  /* alpha, beta: double pairs of cores with fewer than ten members,
+                              read directly by the row product beside the core-grouped ones */
 } sbdg_stats;
 
 /* Context on a device.  Replaces the communicator set-up of TaskCommunicator

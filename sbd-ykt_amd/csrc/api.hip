@@ -62,6 +62,7 @@ struct sbdg_ctx {
   Table Sa, Da, Sb, Db;
   SpmmTable Ma, Mb;
   CoreTable Ca, Cb;       // same-spin doubles by core (used when use_ca / use_cb)
+  Table Xa, Xb;           // the pairs of the small cores, read directly (with Ca / Cb)
   bool use_ca = false, use_cb = false;
   DBuf<double> Ja, Jb;    // occupation tables of the alpha / beta strings
   DBuf<float> Ja32, Jb32;
@@ -173,17 +174,20 @@ void build(sbdg_ctx* c) {
   const int nel_a = na ? __builtin_popcountll(c->a[0]) : 0, nel_b = nb ? __builtin_popcountll(c->b[0]) : 0;
   c->use_ca = c->use_cb = false;
   if (c->opt.doubles != 1) {
-    build_cores(c->d_a.p, na, L, nel_a, c->Ca, c->st);
-    build_cores(c->d_b.p, nb, L, nel_b, c->Cb, c->st);
-    // auto: the core form where it moves fewer rows and one 512-column pass of its
+    build_cores(c->d_a.p, na, L, nel_a, c->eri.data(), c->Ca, c->Xa, c->st);
+    build_cores(c->d_b.p, nb, L, nel_b, c->eri.data(), c->Cb, c->Xb, c->st);
+    // auto: the core form (its workspace for the larger cores, the pairs of the smaller
+    // ones read directly) where it moves fewer rows and one 512-column pass of its
     // workspace fits core_gib; otherwise the excitation-table form
     const size_t cap = (size_t)(c->opt.core_gib * 1073741824.0);
     auto fits = [&](const CoreTable& ct) { return (size_t)ct.nmem * CORE_COLS * sizeof(double) <= cap; };
-    c->use_ca = c->opt.doubles == 2 || (c->Da.nnz > 4 * c->Ca.nmem && fits(c->Ca));
-    c->use_cb = c->opt.doubles == 2 || (c->Db.nnz > 4 * c->Cb.nmem && fits(c->Cb));
+    c->use_ca = c->opt.doubles == 2 || (c->Da.nnz > 4 * c->Ca.nmem + c->Xa.nnz && fits(c->Ca));
+    c->use_cb = c->opt.doubles == 2 || (c->Db.nnz > 4 * c->Cb.nmem + c->Xb.nnz && fits(c->Cb));
   }
-  build_spmm(c->Sa, c->Da, c->Ma, L, c->st, !c->use_ca);
-  build_spmm(c->Sb, c->Db, c->Mb, L, c->st, !c->use_cb);
+  // row products: the singles with the doubles of the excitation table, or with the direct
+  // pairs of the small cores when the doubles go by core
+  build_spmm(c->Sa, c->use_ca ? c->Xa : c->Da, c->Ma, L, c->st, !c->use_ca || c->Xa.nnz > 0);
+  build_spmm(c->Sb, c->use_cb ? c->Xb : c->Db, c->Mb, L, c->st, !c->use_cb || c->Xb.nnz > 0);
   // occupation parts of the singles: beta occupations for the alpha product (columns of
   // c), alpha occupations for the beta product (columns of c^T)
   const int64_t LL1 = (int64_t)L * L + 1;
@@ -229,6 +233,8 @@ void build(sbdg_ctx* c) {
   c->stats.nnz[3] = c->Db.nnz;
   c->stats.core_members[0] = c->use_ca ? c->Ca.nmem : 0;
   c->stats.core_members[1] = c->use_cb ? c->Cb.nmem : 0;
+  c->stats.core_direct[0] = c->use_ca ? c->Xa.nnz : 0;
+  c->stats.core_direct[1] = c->use_cb ? c->Xb.nnz : 0;
   c->stats.terms = N + (c->Sa.nnz + c->Da.nnz) * nb + (c->Sb.nnz + c->Db.nnz) * na + c->Sa.nnz * c->Sb.nnz;
 }
 

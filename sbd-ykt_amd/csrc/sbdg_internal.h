@@ -128,7 +128,9 @@ struct EllTable {
 // in the separated order is eps_x * eps_y * W[P_x][P_y], with P_x = (a < b) the pair added
 // to k to make x, eps_x = <x| c+_a c+_b |k> and W[(A,B)][(I,J)] = (AI|BJ) - (AJ|BI)
 // (zero when the pairs overlap).  Members are stored core by core (only cores with at
-// least two members); row slots list, per string, the member entries it owns.
+// least CORE_KMIN members); row slots list, per string, the member entries it owns.  The
+// pairs of the smaller cores (two members and up) are cheaper read directly: they go to a
+// CSR table of their own that the row-streaming product merges with the singles.
 struct CoreTable {
   int64_t ncore = 0, nmem = 0, nblk = 0;
   int maxm = 0;
@@ -145,12 +147,21 @@ struct CoreTable {
 constexpr int CORE_XB = 16;     // output members per product block
 constexpr int CORE_COLS = 512;  // column granularity of the core-product passes
 constexpr int CORE_WORK = 256;  // member rows streamed per core-product workgroup (target)
+// Smallest core taken through the workspace: a core of k members costs about 3k row
+// transfers there (member rows in, Z out, Z gathered) and at most k(k-1) read directly,
+// fewer where members' pairs overlap, and the direct reads of one core's member rows hit
+// in cache.  FP64 sigma at norb=36, 20,000^2 strings with the cut at 2 (all through the
+// workspace) / 4 / 6 / 8 / 10 / 12 / 17 members: 204.2 / 198.9 / 194.8 / 189.9 / 190.5 /
+// 191.9 / 193.0 ms.
+constexpr int CORE_KMIN = 10;
 
 // kernels (tables.hip)
 void build_table(const uint64_t* d_str, int64_t n, int L, const double* d_h1, const double* d_eri,
                  Table& t_single, Table& t_double, hipStream_t st);
 void build_spmm(const Table& s, const Table& d, SpmmTable& m, int L, hipStream_t st, bool with_doubles = true);
-void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct, hipStream_t st);
+// h_eri: the integrals on the host (full L^4), for the values of the direct pairs
+void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, const double* h_eri, CoreTable& ct,
+                 Table& direct, hipStream_t st);
 // J[(p+1)*n + r] = sum_{o in occ(s[r])} eri[p*L^2 + o*L + o] for p < L^2 (row 0 zero):
 // the occupation part of the other spin's singles, (ai|oo) summed over the string's orbitals
 void build_occ_table(const uint64_t* d_str, int64_t n, int L, const double* d_eri, double* J, float* J32,

@@ -698,12 +698,26 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
   SBDG_CHECK(hipMemcpy(e.ent.p, ent.data(), sizeof(uint32_t) * ent.size(), hipMemcpyHostToDevice));
 }
 
-void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct, hipStream_t st) {
+void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, const double* h_eri, CoreTable& ct,
+                 Table& direct, hipStream_t st) {
   ct.ncore = ct.nmem = ct.nblk = 0;
   ct.maxm = 0;
   const int64_t npp = (int64_t)nel * (nel - 1) / 2;
   const int64_t total = n * npp;
   std::vector<int32_t> start(1, 0), row, pair, blk, blk_x0, grp, slot_ptr(n + 1, 0), slot;
+  // direct pairs of the small cores: (row, column, value), value eps_x eps_y W[P_x][P_y]
+  struct Direct {
+    int32_t r, c;
+    double v;
+  };
+  std::vector<Direct> dp;
+  auto unpair = [](int P, int& lo, int& hi) {
+    int b = 1;
+    while ((b + 1) * b / 2 <= P) ++b;
+    hi = b;
+    lo = P - b * (b - 1) / 2;
+  };
+  auto eri4h = [&](int p, int q, int r, int t) { return h_eri[(((int64_t)p * L + q) * L + r) * L + t]; };
   if (total > 0) {
     if (total >= (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "too many core memberships");
     DBuf<uint64_t> key, val, key2, val2;
@@ -749,7 +763,19 @@ void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct
             break;
           }
       const int64_t m = (int64_t)keep.size();
-      if (m >= 2) {
+      if (m >= 2 && m < CORE_KMIN) {
+        for (int64_t x = 0; x < m; ++x)
+          for (int64_t y = 0; y < m; ++y) {
+            const uint64_t vx = hv[keep[x]], vy = hv[keep[y]];
+            if (x == y || (pair_mask(vx) & pair_mask(vy))) continue;  // W = 0
+            int A, B, I, J;
+            unpair((int)((vx >> 32) & 0x7FFFFFFFull), A, B);
+            unpair((int)((vy >> 32) & 0x7FFFFFFFull), I, J);
+            double v = eri4h(A, I, B, J) - eri4h(A, J, B, I);
+            if ((vx >> 63) != (vy >> 63)) v = -v;
+            dp.push_back({(int32_t)(vx & 0xFFFFFFFFull), (int32_t)(vy & 0xFFFFFFFFull), v});
+          }
+      } else if (m >= 2) {
         const int32_t c = (int32_t)(start.size() - 1);
         for (int64_t x0 = 0; x0 < m; x0 += CORE_XB) {
           blk.push_back(c);
@@ -780,6 +806,35 @@ void build_cores(const uint64_t* d_str, int64_t n, int L, int nel, CoreTable& ct
     slot.resize(row.size());
     std::vector<int32_t> fillp(slot_ptr.begin(), slot_ptr.end() - 1);
     for (size_t q = 0; q < row.size(); ++q) slot[fillp[row[q]]++] = (int32_t)q;
+  }
+  // the direct pairs as a CSR table (rows ascending; any order inside a row)
+  {
+    std::vector<int64_t> rp(n + 1, 0);
+    for (const Direct& d : dp) ++rp[d.r + 1];
+    for (int64_t r = 0; r < n; ++r) rp[r + 1] += rp[r];
+    std::vector<int32_t> dc(dp.size());
+    std::vector<double> dv(dp.size());
+    std::vector<float> dv32(dp.size());
+    std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+    for (const Direct& d : dp) {
+      const int64_t q = fill[d.r]++;
+      dc[q] = d.c;
+      dv[q] = d.v;
+      dv32[q] = (float)d.v;
+    }
+    direct.n = n;
+    direct.nnz = (int64_t)dp.size();
+    direct.rowptr.alloc(n + 1);
+    direct.col.alloc(std::max<size_t>(dp.size(), 1));
+    direct.exc.alloc(1);
+    direct.val.alloc(std::max<size_t>(dp.size(), 1));
+    direct.val32.alloc(std::max<size_t>(dp.size(), 1));
+    SBDG_CHECK(hipMemcpy(direct.rowptr.p, rp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    if (!dp.empty()) {
+      SBDG_CHECK(hipMemcpy(direct.col.p, dc.data(), sizeof(int32_t) * dp.size(), hipMemcpyHostToDevice));
+      SBDG_CHECK(hipMemcpy(direct.val.p, dv.data(), sizeof(double) * dp.size(), hipMemcpyHostToDevice));
+      SBDG_CHECK(hipMemcpy(direct.val32.p, dv32.data(), sizeof(float) * dp.size(), hipMemcpyHostToDevice));
+    }
   }
   ct.ncore = (int64_t)start.size() - 1;
   ct.nmem = (int64_t)row.size();

@@ -46,7 +46,7 @@ class Stats(C.Structure):
         ("iterations", C.c_int), ("sigma_fp64", C.c_int), ("sigma_fp32", C.c_int), ("converged", C.c_int),
         ("t_tables", C.c_double), ("t_davidson", C.c_double), ("t_sigma", C.c_double),
         ("t_final", C.c_double), ("t_rdm", C.c_double), ("terms", C.c_int64), ("nnz", C.c_int64 * 4),
-        ("core_members", C.c_int64 * 2),
+        ("core_members", C.c_int64 * 2), ("core_direct", C.c_int64 * 2),
     ]
 
 
@@ -189,9 +189,10 @@ class Solver:
     def stats(self) -> dict:
         s = Stats()
         self._chk(lib().sbdg_get_stats(self._c, C.byref(s)))
-        d = {k: getattr(s, k) for k, _ in Stats._fields_ if k not in ("nnz", "core_members")}
+        d = {k: getattr(s, k) for k, _ in Stats._fields_ if k not in ("nnz", "core_members", "core_direct")}
         d["nnz"] = list(s.nnz)
         d["core_members"] = list(s.core_members)
+        d["core_direct"] = list(s.core_direct)
         return d
 
     def trajectory(self):
