@@ -75,3 +75,26 @@ def workload_small(L, na_e, nb_e, na, nb, seed):
     a = syn.sample_strings(L, na_e, na, rng) if na else syn.full_space(L, na_e)
     b = syn.sample_strings(L, nb_e, nb, rng) if nb else syn.full_space(L, nb_e)
     return dict(norb=L, core=0.25, h1=h1, eri=eri, astr=a, bstr=b)
+
+
+def core_split(strs, kmin=10):
+    """Brute-force split of the same-spin doubles by core: (memberships of the cores with at
+    least kmin members, ordered pairs of the smaller cores).  A member counts when its pair
+    is disjoint from another member's pair; a pair counts when the two pairs are disjoint."""
+    members = {}
+    for s in strs:
+        s = int(s)
+        occ = [i for i in range(64) if s >> i & 1]
+        for i, a in enumerate(occ):
+            for b in occ[i + 1:]:
+                members.setdefault(s & ~(1 << a) & ~(1 << b), []).append((1 << a) | (1 << b))
+    nmem = direct = 0
+    for ms in members.values():
+        keep = [m for j, m in enumerate(ms) if any((m & o) == 0 for t, o in enumerate(ms) if t != j)]
+        if len(keep) < 2:
+            continue
+        if len(keep) >= kmin:
+            nmem += len(keep)
+        else:
+            direct += sum(1 for j, x in enumerate(keep) for t, y in enumerate(keep) if t != j and (x & y) == 0)
+    return nmem, direct

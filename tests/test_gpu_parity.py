@@ -12,7 +12,8 @@ import bruteforce as bf
 import oracle
 import sbdg
 from conftest import load_golden
-from helpers import SMALL_CASES, dataset, energy_from_rdm, rdm_close, small_case, tables_from_ref, workload_small
+from helpers import (SMALL_CASES, core_split, dataset, energy_from_rdm, rdm_close, small_case, tables_from_ref,
+                     workload_small)
 from sbdg import fcidump as fio
 from sbdg import synthetic as syn
 
@@ -166,6 +167,14 @@ def test_edge_string_sets_vs_bruteforce(L, na_e, nb_e, na, nb):
 def test_sigma_medium_vs_oracle_and_symmetry(doubles):
     case = medium_case()
     s, p = solver(case, doubles=doubles), oproblem(case)
+    if doubles == 2:  # both forms carry pairs here: cores of 10+ members and smaller ones
+        s.build()
+        st = s.stats()
+        ea, eb = core_split(case["astr"]), core_split(case["bstr"])
+        assert st["core_members"] == [ea[0], eb[0]] and st["core_direct"] == [ea[1], eb[1]]
+        assert min(ea) > 0 and min(eb) > 0
+        nd = st["nnz"]
+        assert ea[1] < nd[1] and eb[1] < nd[3]  # the rest of the doubles go through the workspace
     rng = np.random.default_rng(2)
     x, y = rng.normal(size=s.ndet), rng.normal(size=s.ndet)
     sx, ox = s.sigma(x), p.sigma(x)

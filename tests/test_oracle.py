@@ -15,7 +15,8 @@ import pytest
 
 import bruteforce as bf
 import oracle
-from helpers import SMALL_CASES, dataset, energy_from_rdm, rdm_close, small_case, tables_from_ref, workload_small
+from helpers import (SMALL_CASES, core_split, dataset, energy_from_rdm, rdm_close, small_case, tables_from_ref,
+                     workload_small)
 from sbdg import fcidump as fio
 from sbdg import synthetic as syn
 
@@ -143,6 +144,20 @@ def test_table_counts_closed_form(L, n):
         d = np.array([bin(int(s[i]) ^ int(x)).count("1") for x in s])
         assert np.array_equal(cS[rpS[i]:rpS[i + 1]], np.nonzero(d == 2)[0])
         assert np.array_equal(cD[rpD[i]:rpD[i + 1]], np.nonzero(d == 4)[0])
+
+
+def test_core_split_holds_every_double_once():
+    """Each same-spin double pair lies in exactly one (n-2)-electron core: with no core taken
+    through the workspace, the brute-force core split's direct pairs are the oracle's
+    doubles table, entry for entry in number (the split the GPU statistics are checked
+    against)."""
+    case = workload_small(26, 5, 5, 400, 300, 31)
+    p = problem(case)
+    for strs, which in ((case["astr"], 1), (case["bstr"], 3)):
+        _, col = p.table(which)
+        assert core_split(strs, kmin=1 << 30) == (0, len(col))
+        nmem, direct = core_split(strs)
+        assert 0 < direct < len(col) and nmem > 0
 
 
 def test_hii_closed_form():
