@@ -245,6 +245,19 @@ def test_sigma_c2_full_size_rows_and_symmetry():
     s.close()
 
 
+@pytest.mark.parametrize("L", [45, 46, 48])
+def test_sigma_slice_copies_vs_oracle(L):
+    """The alpha-beta kernel stages its integral slices twice (the schedule picks a copy per
+    entry) while an entry's 12-bit slice field reaches the second copy, up to norb=45, and
+    once from norb=46: sigma against the oracle on both sides of that limit."""
+    case = workload_small(L, 4, 3, 300, 400, 61)
+    s, p = solver(case), oproblem(case)
+    x = np.random.default_rng(1).normal(size=s.ndet)
+    sx, ox = s.sigma(x), p.sigma(x)
+    assert np.abs(sx - ox).max() <= 1e-12 * np.abs(ox).max()
+    assert np.abs(s.sigma(x, precision=1) - ox).max() <= 2e-5 * np.abs(ox).max()
+
+
 @pytest.mark.parametrize("overlap", [1, 0])
 def test_sigma_wide_beta_set_vs_oracle(overlap):
     """30,000 beta strings (norb=24, 5 electrons) against 12 alpha strings: the alpha-beta
