@@ -40,7 +40,10 @@ constexpr int TS = 256;  // threads of the row-streaming product
 //             + (ZG ? sum_{s in slots(r)} Z[slot[s], c - cb] : 0)
 // jai[q] = +-(p_q) selects the row of the column strings' occupation table J that carries
 // the other-spin occupation part of a single (row 0: zero, for doubles).  Z holds the
-// core-grouped doubles of the same column range (k_core_prod).
+// core-grouped doubles of the same column range (k_core_prod).  Workgroups are dealt to the
+// 8 XCDs in turn and workgroup L takes item (L % 8) * per + L / 8 (per = ceil(items / 8)):
+// each XCD walks a contiguous run of rows, whose strings share many excitation partners,
+// so the rows of X they read meet in that XCD's L2.
 template <class T, bool DIAG, bool VECTOR, bool ZG>
 __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64_t cb, int64_t ce,
                                              const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -51,7 +54,11 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
                                              const T* __restrict__ Z, int64_t zcols) {
   constexpr int VN = V16<T>::n;
   using VT = typename V16<T>::t;
-  const int64_t bid = blockIdx.x;
+  const int64_t ntl = (ce - cb + TS * VN - 1) / (TS * VN);
+  const int64_t items = ntl * nrows;
+  const int64_t per = (items + 7) / 8;
+  const int64_t bid = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;  // each XCD: a contiguous run of rows
+  if (bid >= items) return;
   const int64_t tile = bid / nrows;
   const int64_t r = bid - tile * nrows;
   const int64_t c0 = cb + tile * (int64_t)(TS * VN) + (int64_t)threadIdx.x * VN;
@@ -519,7 +526,7 @@ void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, c
   if (nrows == 0 || ce <= cb) return;
   constexpr int VN = V16<T>::n;
   const int64_t ntiles = (ce - cb + TS * VN - 1) / (TS * VN);
-  const int64_t nblk = ntiles * nrows;
+  const int64_t nblk = (ntiles * nrows + 7) / 8 * 8;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
   const T* val = nullptr;
   if constexpr (sizeof(T) == 8) val = reinterpret_cast<const T*>(tb.val.p);
