@@ -1,9 +1,12 @@
 """Host-side checks that need no GPU: the C-ABI library loads and exports every symbol its
-header declares, the FCIDUMP / string loaders keep the reference's semantics, and the
-synthetic workload generator is deterministic and follows its specification."""
+header declares, the FCIDUMP / string loaders keep the reference's semantics, the synthetic
+workload generator is deterministic and follows its specification, and the benchmark
+refuses more than one GPU."""
 import ctypes as C
 import os
 import re
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -116,3 +119,11 @@ def test_generator_deterministic_and_levels():
     assert (lv == 0).sum() == 1 and (lv == 1).sum() <= 105 and (lv == 2).sum() > (lv == 4).sum()
     with pytest.raises(ValueError):
         syn.sample_strings(6, 3, 21, np.random.default_rng(0))
+
+
+def test_bench_measures_one_gpu_only():
+    """The benchmark has no multi-GPU path: --gpus above 1 exits at once with a message,
+    before anything touches a device."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
