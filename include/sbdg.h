@@ -86,6 +86,9 @@ typedef struct {
                               excitation-table form is used for that spin) */
   int64_t core_direct[2];  /* alpha, beta: double pairs of cores with fewer than ten members,
                               read directly by the row product beside the core-grouped ones */
+  int64_t mem_now;      /* device bytes the library holds when the stats are read (all
+                           contexts of the process) */
+  int64_t mem_peak;     /* their high-water mark since the start of the last build or run */
 } sbdg_stats;
 
 /* Context on a device.  Replaces the communicator set-up of TaskCommunicator
@@ -168,6 +171,17 @@ int sbdg_carryover(sbdg_ctx *ctx, double ratio, double threshold, uint64_t *out,
  * the energy at every step whose energy is an FP64 Rayleigh quotient (every step of the
  * FP64 path; every refresh of the mixed path).  Writes min(cap, *n) entries. */
 int sbdg_get_trajectory(const sbdg_ctx *ctx, double *t, double *e, int cap, int *n);
+
+/* Planned device bytes of a run, computed without a device: out[0] resident after the
+ * build (an upper bound where table padding depends on the strings), out[1] the Davidson
+ * vectors for opt's precision and max_nb (the basis, its sigma vectors, W, R, T; mixed:
+ * the anchor's FP64 sigma and the FP32 input vector; sigma's scratch borrows W, R and T),
+ * out[2] the RDMs' transient peak and outputs.  nnz, core_members and core_direct as
+ * sbdg_stats reports them after a build (core_members[s] == 0: that spin's doubles go by
+ * excitation table).  No counterpart in the reference, which sizes its Davidson memory by
+ * hand (data/h2o/README.md:1). */
+int sbdg_memory_plan(const sbdg_options *opt, int norb, int64_t na, int64_t nb, const int64_t *nnz,
+                     const int64_t *core_members, const int64_t *core_direct, int64_t *out);
 
 /* ---- lower-level entry points (parity tests and benchmarks) ---- */
 

@@ -380,12 +380,13 @@ void orc_sigma(const orc_problem *p, const double *hii, const double *T, double 
 
 /* sigma restricted to bra rows [ia0, ia1): the bounded CPU sample used as a timing
  * baseline.  Writes Wb rows ia0..ia1-1 (Wb indexed like the full vector); hii may be
- * NULL, then the diagonal is evaluated in place. */
+ * NULL, then the diagonal is evaluated in place.  Every (bra row, bra column) element is
+ * independent, so the threads share the elements in runs of 64 columns. */
 void orc_sigma_rows(const orc_problem *p, const double *hii, const double *T, double *Wb,
                     int64_t ia0, int64_t ia1) {
   const int64_t nb = p->nb;
   const int L = p->L;
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for collapse(2) schedule(dynamic, 64)
   for (int64_t ia = ia0; ia < ia1; ++ia) {
     for (int64_t ib = 0; ib < nb; ++ib) {
       u128 I = interleave(p->a[ia], p->b[ib], L);

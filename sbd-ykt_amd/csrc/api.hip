@@ -242,17 +242,89 @@ void ensure_built(sbdg_ctx* c) {
   if (!c->built) build(c);
 }
 
-// device scratch for one sigma of precision T
-void sigma64(sbdg_ctx* c, const double* x, double* y, hipEvent_t* ev = nullptr) {
-  const int64_t N = (int64_t)c->a.size() * c->b.size();
-  c->scr64.alloc(std::max<int64_t>(3 * N, 1));
-  sigma_apply<double>(plan(c, false), x, y, c->scr64.p, c->st, ev);
+// sigma with explicit scratch vectors (the Davidson loop lends it vectors that are idle
+// while sigma runs), or with the context's own scratch (standalone sigma and benchmarks)
+void sigma64(sbdg_ctx* c, const double* x, double* y, const SigmaScratch<double>& scr, hipEvent_t* ev = nullptr) {
+  sigma_apply<double>(plan(c, false), x, y, scr, c->st, ev);
 }
 
-void sigma32(sbdg_ctx* c, const float* x, float* y, hipEvent_t* ev = nullptr) {
+void sigma32(sbdg_ctx* c, const float* x, float* y, const SigmaScratch<float>& scr, hipEvent_t* ev = nullptr) {
+  sigma_apply<float>(plan(c, true), x, y, scr, c->st, ev);
+}
+
+SigmaScratch<double> own_scratch64(sbdg_ctx* c) {
+  const int64_t N = (int64_t)c->a.size() * c->b.size();
+  c->scr64.alloc(std::max<int64_t>(3 * N, 1));
+  return {c->scr64.p, c->scr64.p + N, c->scr64.p + 2 * N};
+}
+
+SigmaScratch<float> own_scratch32(sbdg_ctx* c) {
   const int64_t N = (int64_t)c->a.size() * c->b.size();
   c->scr32.alloc(std::max<int64_t>(3 * N, 1));
-  sigma_apply<float>(plan(c, true), x, y, c->scr32.p, c->st, ev);
+  return {c->scr32.p, c->scr32.p + N, c->scr32.p + 2 * N};
+}
+
+// Device vectors of the Davidson loop for N determinants (elements of 8 bytes unless
+// stated): the basis V (nbmax), the sigma of every basis vector (HV, nbmax; in the mixed
+// path HV32, nbmax FP32), W, R, T, and in the mixed path the anchor's FP64 sigma HX and the
+// FP32 input of the next sigma c32 (FP32).  sigma's three scratch vectors are W, R and T
+// (free while sigma runs), so they add nothing.
+int64_t davidson_bytes(const sbdg_options& o, int64_t N) {
+  const int64_t nbmax = std::max(2, std::min(o.max_nb, MAXSUB));
+  if (o.precision == 1) return N * (nbmax * 8 + nbmax * 4 + 3 * 8 + 8 + 4);
+  return N * (nbmax * 8 * 2 + 3 * 8);
+}
+
+// Resident device bytes after the build (an upper bound where a table's padding depends on
+// the strings): integrals, strings, excitation tables, row-product tables, core tables and
+// workspace, occupation tables, the alpha-beta tables, the diagonal.  nnz: alpha singles,
+// alpha doubles, beta singles, beta doubles; mem/dir: core memberships and directly read
+// pairs per spin (0 members: that spin's doubles go by excitation table).
+int64_t build_bytes(const sbdg_options& o, int L, int64_t na, int64_t nb, const int64_t* nnz, const int64_t* mem,
+                    const int64_t* dir) {
+  const int64_t L2 = (int64_t)L * L, L4 = L2 * L2, N = na * nb, npair = (int64_t)L * (L - 1) / 2;
+  int64_t b = 8 * L2 + 12 * L4 + 8 * (na + nb) + 12 * N;  // h1, eri FP64+FP32, strings, hii FP64+FP32
+  bool any_core = false;
+  int64_t zwant = 0;
+  const int64_t ns[2] = {na, nb}, nc[2] = {nb, na};
+  for (int sp = 0; sp < 2; ++sp) {
+    const int64_t n = ns[sp], S = nnz[2 * sp], D = nnz[2 * sp + 1];
+    const bool core = mem[sp] > 0;
+    b += 2 * 8 * (n + 1) + 20 * (S + D);                          // singles and doubles tables
+    b += 8 * (n + 1) + 20 * (S + (core ? dir[sp] : D));          // row-product table
+    b += 12 * (L2 + 1) * n;                                        // occupation tables
+    if (core) {
+      any_core = true;
+      b += 4 * (n + 1) + 28 * mem[sp] + 8 * (n + 1) + 16 * dir[sp];  // members, blocks, slots; direct pairs
+      zwant = std::max(zwant, mem[sp] * ((nc[sp] + CORE_COLS - 1) / CORE_COLS * CORE_COLS) * 8);
+    }
+  }
+  if (any_core) b += 12 * npair * npair + std::min<int64_t>(zwant, (int64_t)(o.core_gib * 1073741824.0));
+  // alpha-beta tables: RDM pack and chunk pointers; ELL entries (a group pads to its longest
+  // list, at most the group's whole sum, plus three rows), lane maps, stream offsets
+  const int64_t Sb = nnz[2];
+  const int64_t ach = std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - L2));
+  b += 4 * Sb + 4 * (nb * ((nb + ach - 1) / ach + 1) + 1);
+  const int64_t nch = (nb + std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32) - 1) /
+                      std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32);
+  const int64_t groups = (nb + 63) / 64 + 1024;  // tiles round the lane positions up
+  b += 4 * (64 * Sb + 192 * groups * nch) + 4 * nch * groups * 64 + 16 * groups * nch;
+  return b;
+}
+
+// Transient bytes of the RDMs beyond the resident build and the vectors (which the RDMs
+// borrow): the largest of the per-table passes, plus the outputs 2 L^2 + 4 L^4 doubles.
+int64_t rdm_bytes(int L, const int64_t* nnz) {
+  const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
+  int64_t peak = 0;
+  for (int sp = 0; sp < 2; ++sp) {
+    const int64_t S = nnz[2 * sp], D = nnz[2 * sp + 1];
+    peak = std::max(peak, 8 * S + 8 * S * L + 16 * S + 16 * S + 4 * (L2 + 1));  // gram, keys, sort
+    peak = std::max(peak, 8 * D + 32 * D + 16 * D);                            // gram, keys, scan, sort
+  }
+  const int64_t Sa = nnz[0];
+  peak = std::max(peak, 8 * Sa + 16 * Sa + 16 * Sa + 8 * (Sa / 32 + L2 + 1) * L2);  // alpha-beta partials
+  return peak + 8 * (2 * L2 + 4 * L4);
 }
 
 struct Small {  // device scalars carved from one buffer
@@ -272,25 +344,28 @@ Small carve(sbdg_ctx* c) {
   return s;
 }
 
-void initial_vector(sbdg_ctx* c, double* W) {
+// The start vector, formed on the device in T (reference phases), normalised, then
+// converted to the separated phases into W.  init 0: the first determinant (BasisInitVector,
+// tpb/davidson.h:130-133); 1: values in [-1, 1) from a counter-based hash (splitmix64 of
+// seed and index); 2: the caller's vector.
+void initial_vector(sbdg_ctx* c, Small& s, double* W) {
   const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
-  std::vector<double> h(N, 0.0);
   if (c->opt.init == 2) {
     if (!c->have_user_vec) throw Error(SBDG_ERR_STATE, "init=2 but no vector set");
-    h = c->user_vec;
+    SBDG_CHECK(hipMemcpyAsync(c->T.p, c->user_vec.data(), sizeof(double) * N, hipMemcpyHostToDevice, c->st));
   } else if (c->opt.init == 1) {
-    std::mt19937_64 g(c->opt.seed);
-    std::uniform_real_distribution<double> d(-1.0, 1.0);
-    for (auto& x : h) x = d(g);
+    fill_test_vector(c->T.p, N, c->opt.seed, c->st);
   } else {
-    h[0] = 1.0;  // BasisInitVector (tpb/davidson.h:130-133): the first determinant
+    const double one = 1.0;
+    SBDG_CHECK(hipMemsetAsync(c->T.p, 0, sizeof(double) * N, c->st));
+    SBDG_CHECK(hipMemcpyAsync(c->T.p, &one, sizeof(double), hipMemcpyHostToDevice, c->st));
   }
+  multi_dot_impl(c->T.p, N, 1, c->T.p, N, s.dots, s.part, c->st);
   double nn = 0.0;
-  for (double x : h) nn += x * x;
+  SBDG_CHECK(hipMemcpyAsync(&nn, s.dots, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  SBDG_CHECK(hipStreamSynchronize(c->st));
   if (!(nn > 0.0) || !std::isfinite(nn)) throw Error(SBDG_ERR_ARG, "initial vector has zero or non-finite norm");
-  const double f = 1.0 / std::sqrt(nn);
-  for (auto& x : h) x *= f;
-  SBDG_CHECK(hipMemcpyAsync(c->T.p, h.data(), sizeof(double) * N, hipMemcpyHostToDevice, c->st));
+  scale_impl(c->T.p, s.dots, N, c->st);
   sign_transform(c->d_a.p, na, c->d_b.p, nb, c->T.p, W, c->st);  // reference phases -> separated
   SBDG_CHECK(hipStreamSynchronize(c->st));
 }
@@ -339,17 +414,38 @@ void davidson(sbdg_ctx* c) {
   const int nbmax = std::max(2, std::min(o.max_nb, MAXSUB));
   const int keep = std::max(1, std::min(o.restart_keep, nbmax - 1));
   const int max_steps = std::max(1, o.max_it) * nbmax;
+  // only the vectors this precision uses stay allocated (davidson_bytes); the standalone
+  // sigma's scratch and the benchmark vectors are released first
+  c->scr64.free();
+  c->scr32.free();
+  c->bx64.free();
+  c->by64.free();
+  c->bx32.free();
+  c->by32.free();
+  if (mixed) {
+    c->HV.free();
+  } else {
+    c->HV32.free();
+    c->HX.free();
+    c->c32.free();
+  }
   c->V.alloc((size_t)nbmax * N);
   c->W.alloc(N);
   c->R.alloc(N);
   c->T.alloc(N);
-  c->HX.alloc(N);
   if (mixed) {
     c->HV32.alloc((size_t)nbmax * N);
+    c->HX.alloc(N);
     c->c32.alloc(N);
   } else {
     c->HV.alloc((size_t)nbmax * N);
   }
+  // sigma's scratch: W, R and T hold nothing that is needed while a sigma runs (W is formed
+  // again by the next Ritz step, R was consumed by the last expansion, T is a staging
+  // vector); an FP32 sigma fits its three FP32 vectors into T and R
+  const SigmaScratch<double> scr64{c->T.p, c->R.p, c->W.p};
+  const SigmaScratch<float> scr32{reinterpret_cast<float*>(c->T.p), reinterpret_cast<float*>(c->T.p) + N,
+                                  reinterpret_cast<float*>(c->R.p)};
   Small s = carve(c);
   // device scalar slots: norms[0] |W|^2, [1] |R|^2 (combine), [2] theta and [3] |R|^2 used
   // by the next expansion, [6] the constant 1, [8] |t|^2
@@ -381,8 +477,9 @@ void davidson(sbdg_ctx* c) {
     c->stats.terms = keep_st.terms;
     for (int k = 0; k < 4; ++k) c->stats.nnz[k] = keep_st.nnz[k];
     for (int k = 0; k < 2; ++k) c->stats.core_members[k] = keep_st.core_members[k];
+    for (int k = 0; k < 2; ++k) c->stats.core_direct[k] = keep_st.core_direct[k];
   }
-  initial_vector(c, c->V.p);
+  initial_vector(c, s, c->V.p);
   h2d(one_dev, 1.0);
   const double tstart = now();
   c->traj_t.clear();
@@ -463,7 +560,8 @@ void davidson(sbdg_ctx* c) {
   // mixed: FP64 sigma of basis vector 0 (= current Ritz vector): exact row/column 0 of
   // the projected matrix, exact Rayleigh quotient and residual
   auto refresh = [&]() {
-    timed([&] { sigma64(c, c->V.p, c->HX.p); });
+    // W holds a copy of V_0 here (restored below), so it may serve as scratch
+    timed([&] { sigma64(c, c->V.p, c->HX.p, scr64); });
     c->stats.sigma_fp64++;
     multi_dot_impl(c->V.p, N, v, c->HX.p, N, s.dots, s.part, c->st);
     SBDG_CHECK(hipMemcpyAsync(dots.data(), s.dots, sizeof(double) * v, hipMemcpyDeviceToHost, c->st));
@@ -518,7 +616,7 @@ void davidson(sbdg_ctx* c) {
     if (mixed) {
       timed([&] {
         if (c32_of != h) cast_d2f(c->V.p + (int64_t)h * N, c->c32.p, N, c->st);
-        sigma32(c, c->c32.p, c->HV32.p + (int64_t)h * N);
+        sigma32(c, c->c32.p, c->HV32.p + (int64_t)h * N, scr32);
       });
       c32_of = -1;
       c->stats.sigma_fp32++;
@@ -531,7 +629,7 @@ void davidson(sbdg_ctx* c) {
       if (have_anchor)
         SBDG_CHECK(hipMemcpyAsync(&dots[0], s.dots + h + 1, sizeof(double), hipMemcpyDeviceToHost, c->st));
     } else {
-      timed([&] { sigma64(c, c->V.p + (int64_t)h * N, c->HV.p + (int64_t)h * N); });
+      timed([&] { sigma64(c, c->V.p + (int64_t)h * N, c->HV.p + (int64_t)h * N, scr64); });
       c->stats.sigma_fp64++;
       multi_dot_impl(c->V.p, N, h + 1, c->HV.p + (int64_t)h * N, N, s.dots, s.part, c->st);
       SBDG_CHECK(hipMemcpyAsync(dots.data(), s.dots, sizeof(double) * (h + 1), hipMemcpyDeviceToHost, c->st));
@@ -621,11 +719,15 @@ void davidson(sbdg_ctx* c) {
   c->stats.t_davidson = now() - tstart;
   SBDG_CHECK(hipEventDestroy(e0));
   SBDG_CHECK(hipEventDestroy(e1));
-  // final energy: one more FP64 sigma of the normalised W, then <W|HW> (sbdiag.h:250-273)
+  // final energy: one more FP64 sigma of the normalised W, then <W|HW> (sbdiag.h:250-273);
+  // the sigma vectors of the basis are free now: the result goes to HX (mixed) or HV_0,
+  // the third scratch vector to the FP32 sigma vectors (mixed) or HV_1
   const double tf = now();
-  sigma64(c, c->W.p, c->HX.p);
+  double* hw = mixed ? c->HX.p : c->HV.p;
+  double* third = mixed ? reinterpret_cast<double*>(c->HV32.p) : c->HV.p + N;
+  sigma64(c, c->W.p, hw, SigmaScratch<double>{c->T.p, c->R.p, third});
   c->stats.sigma_fp64++;
-  multi_dot_impl(c->W.p, N, 1, c->HX.p, N, s.dots, s.part, c->st);
+  multi_dot_impl(c->W.p, N, 1, hw, N, s.dots, s.part, c->st);
   double e = 0.0;
   SBDG_CHECK(hipMemcpyAsync(&e, s.dots, sizeof(double), hipMemcpyDeviceToHost, c->st));
   SBDG_CHECK(hipStreamSynchronize(c->st));
@@ -653,7 +755,8 @@ void rdm_or_density(sbdg_ctx* c) {
     P.sb = &c->Sb;
     P.db = &c->Db;
     P.ab = &c->ab64;
-    rdm_compute(P, c->W.p, c->d_one.p, c->d_two.p, c->st);
+    c->T.alloc(na * nb);  // the transposed vector
+    rdm_compute(P, c->W.p, c->T.p, c->d_one.p, c->d_two.p, c->st);
     c->have_rdm = true;
   }
   c->d_dens.alloc(2 * L);
@@ -869,25 +972,41 @@ int sbdg_set_vector(sbdg_ctx* c, const double* w) {
 
 int sbdg_build(sbdg_ctx* c) {
   return guard(c, [&]() {
+    mem_count().reset_peak();
     build(c);
+    c->stats.mem_now = mem_count().now.load();
+    c->stats.mem_peak = mem_count().peak.load();
     return SBDG_OK;
   });
 }
 
 int sbdg_run(sbdg_ctx* c) {
   return guard(c, [&]() {
+    mem_count().reset_peak();
     ensure_built(c);
     const int64_t N = (int64_t)c->a.size() * c->b.size();
     c->T.alloc(N);
     davidson(c);
     rdm_or_density(c);
+    c->stats.mem_now = mem_count().now.load();
+    c->stats.mem_peak = mem_count().peak.load();
     return SBDG_OK;
   });
+}
+
+int sbdg_memory_plan(const sbdg_options* o, int norb, int64_t na, int64_t nb, const int64_t* nnz,
+                     const int64_t* mem, const int64_t* dir, int64_t* out) {
+  if (!o || !nnz || !mem || !dir || !out || norb < 1 || norb > 64 || na < 0 || nb < 0) return SBDG_ERR_ARG;
+  out[0] = build_bytes(*o, norb, na, nb, nnz, mem, dir);
+  out[1] = davidson_bytes(*o, na * nb);
+  out[2] = rdm_bytes(norb, nnz);
+  return SBDG_OK;
 }
 
 int sbdg_get_stats(const sbdg_ctx* c, sbdg_stats* st) {
   if (!c || !st) return SBDG_ERR_ARG;
   *st = c->stats;
+  st->mem_now = mem_count().now.load();
   return SBDG_OK;
 }
 
@@ -1091,13 +1210,14 @@ int sbdg_carryover(sbdg_ctx* c, double ratio, double threshold, uint64_t* out, i
                    double* truncated) {
   return guard(c, [&]() {
     if (!c->solved) return fail(c, SBDG_ERR_STATE, "no solution");
-    const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
-    std::vector<double> w(N);
-    SBDG_CHECK(hipMemcpy(w.data(), c->W.p, sizeof(double) * N, hipMemcpyDeviceToHost));
-    // DiagonalReducedDensityMatrix (tpb/rdmat.h:11-58): row weights of |W|^2
+    const int64_t na = c->a.size(), nb = c->b.size();
+    // DiagonalReducedDensityMatrix (tpb/rdmat.h:11-58): row weights of |W|^2, on the device
     std::vector<double> D(na, 0.0);
-    for (int64_t ia = 0; ia < na; ++ia)
-      for (int64_t ib = 0; ib < nb; ++ib) D[ia] += w[ia * nb + ib] * w[ia * nb + ib];
+    DBuf<double> rs;
+    rs.alloc(std::max<int64_t>(na, 1));
+    row_sumsq(c->W.p, na, nb, rs.p, c->st);
+    SBDG_CHECK(hipMemcpyAsync(D.data(), rs.p, sizeof(double) * na, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
     std::vector<int64_t> idx(na);
     for (int64_t i = 0; i < na; ++i) idx[i] = i;
     std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return D[x] > D[y]; });
@@ -1170,25 +1290,24 @@ int sbdg_sigma(sbdg_ctx* c, const double* x, double* y, int precision) {
     if (precision != 0 && precision != 1) return fail(c, SBDG_ERR_ARG, "precision must be 0 or 1");
     ensure_built(c);
     const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
-    DBuf<double> in, sep, out;
+    DBuf<double> in, out;
     in.alloc(N);
-    sep.alloc(N);
     out.alloc(N);
     SBDG_CHECK(hipMemcpyAsync(in.p, x, sizeof(double) * N, hipMemcpyHostToDevice, c->st));
-    sign_transform(c->d_a.p, na, c->d_b.p, nb, in.p, sep.p, c->st);
+    sign_transform(c->d_a.p, na, c->d_b.p, nb, in.p, in.p, c->st);  // in place: elementwise
     if (precision == 0) {
-      sigma64(c, sep.p, out.p);
+      sigma64(c, in.p, out.p, own_scratch64(c));
     } else {
       DBuf<float> x32, y32;
       x32.alloc(N);
       y32.alloc(N);
-      cast_d2f(sep.p, x32.p, N, c->st);
-      sigma32(c, x32.p, y32.p);
+      cast_d2f(in.p, x32.p, N, c->st);
+      sigma32(c, x32.p, y32.p, own_scratch32(c));
       cast_f2d(y32.p, out.p, N, c->st);
       SBDG_CHECK(hipStreamSynchronize(c->st));
     }
-    sign_transform(c->d_a.p, na, c->d_b.p, nb, out.p, in.p, c->st);
-    SBDG_CHECK(hipMemcpyAsync(y, in.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+    sign_transform(c->d_a.p, na, c->d_b.p, nb, out.p, out.p, c->st);
+    SBDG_CHECK(hipMemcpyAsync(y, out.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
     SBDG_CHECK(hipStreamSynchronize(c->st));
     return SBDG_OK;
   });
@@ -1214,8 +1333,8 @@ int sbdg_bench_sigma(sbdg_ctx* c, int precision, int reps, double* ms) {
     for (auto& e : ev) SBDG_CHECK(hipEventCreate(&e));
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int r = 0; r < reps; ++r) {
-      if (precision == 1) sigma32(c, c->bx32.p, c->by32.p, ev);
-      else sigma64(c, c->bx64.p, c->by64.p, ev);
+      if (precision == 1) sigma32(c, c->bx32.p, c->by32.p, own_scratch32(c), ev);
+      else sigma64(c, c->bx64.p, c->by64.p, own_scratch64(c), ev);
       SBDG_CHECK(hipEventSynchronize(ev[5]));
       float t;
       SBDG_CHECK(hipEventElapsedTime(&t, ev[0], ev[5]));

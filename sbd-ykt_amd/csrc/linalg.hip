@@ -99,8 +99,9 @@ __global__ void __launch_bounds__(RT) k_final(const double* __restrict__ part, i
   }
 }
 
+// in may equal out (each element is read, then written, by one thread)
 __global__ void k_sign(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, int64_t nb,
-                       int64_t N, const double* __restrict__ in, double* __restrict__ out) {
+                       int64_t N, const double* in, double* out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
   const int64_t ia = t / nb, ib = t - ia * nb;
@@ -735,6 +736,11 @@ template void rotate_impl<float>(float*, int64_t, int, const double*, int, int, 
 void syev_device(const double* d_H, int m, int ld, double* d_E, double* d_U, hipStream_t st) {
   if (m > 64) throw Error(SBDG_ERR_LIMIT, "subspace beyond 64");
   hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(64), 0, st, d_H, m, ld, d_E, d_U);
+  SBDG_CHECK(hipGetLastError());
+}
+
+void row_sumsq(const double* W, int64_t na, int64_t nb, double* d_rs, hipStream_t st) {
+  if (na > 0) hipLaunchKernelGGL(k_rowsq, dim3((unsigned)na), dim3(RT), 0, st, W, nb, d_rs);
   SBDG_CHECK(hipGetLastError());
 }
 

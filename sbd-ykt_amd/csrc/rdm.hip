@@ -31,6 +31,19 @@ namespace {
 
 constexpr int RT = 256;
 
+// Sums that many lanes of a workgroup add into one LDS slot are kept as exact 64-bit
+// integer adds of each term's fixed-point image (term * 2^56, rounded to nearest), so their
+// value does not depend on the order in which the lanes arrive.  W has unit norm, so every
+// such sum is bounded by Cauchy-Schwarz: |sum over columns of W[r,c] W[q,c] n_k(c)| <= 1 for
+// the row-pair sums, and <= PCH (32) for the alpha-beta partials of one workgroup, whose
+// beta excitation (b <- j) maps each bra column to at most one ket column.  2^56 leaves a
+// range of +-128 and resolves 1.4e-17 per term.
+constexpr double FX_SCALE = 72057594037927936.0;  // 2^56
+__device__ inline void fx_add(unsigned long long* a, double v) {
+  atomicAdd(a, (unsigned long long)__double2ll_rn(v * FX_SCALE));
+}
+__device__ inline double fx_get(unsigned long long a) { return (double)(long long)a * (1.0 / FX_SCALE); }
+
 __device__ inline double block_sum_d(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -49,13 +62,13 @@ __global__ void __launch_bounds__(RT) k_gram(int64_t ncols, int L, const int64_t
                                              const uint64_t* __restrict__ cstr, double* __restrict__ g,
                                              double* __restrict__ gocc) {
   __shared__ double sh[RT / 64];
-  __shared__ double acc[64];
+  __shared__ unsigned long long acc[64];
   const int64_t r = blockIdx.x;
   const double* xr = X + r * ncols;
   for (int64_t q = rowptr[r]; q < rowptr[r + 1]; ++q) {
     const double* xq = X + (int64_t)col[q] * ncols;
     if (OCC) {
-      if (threadIdx.x < 64) acc[threadIdx.x] = 0.0;
+      if (threadIdx.x < 64) acc[threadIdx.x] = 0ull;
       __syncthreads();
     }
     double s = 0.0;
@@ -67,7 +80,7 @@ __global__ void __launch_bounds__(RT) k_gram(int64_t ncols, int L, const int64_t
         while (y) {
           const int k = __builtin_ctzll(y);
           y &= y - 1;
-          atomicAdd(&acc[k], p);
+          fx_add(&acc[k], p);
         }
       }
     }
@@ -75,7 +88,7 @@ __global__ void __launch_bounds__(RT) k_gram(int64_t ncols, int L, const int64_t
     if (threadIdx.x == 0) g[q] = s;
     if (OCC) {
       __syncthreads();
-      if (threadIdx.x < L) gocc[q * L + threadIdx.x] = acc[threadIdx.x];
+      if (threadIdx.x < L) gocc[q * L + threadIdx.x] = fx_get(acc[threadIdx.x]);
       __syncthreads();
     }
   }
@@ -86,9 +99,9 @@ __global__ void __launch_bounds__(RT) k_rowocc(int64_t ncols, int L, const doubl
                                                const uint64_t* __restrict__ cstr, double* __restrict__ rsq,
                                                double* __restrict__ P) {
   __shared__ double sh[RT / 64];
-  __shared__ double acc[64];
+  __shared__ unsigned long long acc[64];
   const int64_t r = blockIdx.x;
-  if (threadIdx.x < 64) acc[threadIdx.x] = 0.0;
+  if (threadIdx.x < 64) acc[threadIdx.x] = 0ull;
   __syncthreads();
   double s = 0.0;
   for (int64_t cc = threadIdx.x; cc < ncols; cc += RT) {
@@ -100,14 +113,14 @@ __global__ void __launch_bounds__(RT) k_rowocc(int64_t ncols, int L, const doubl
       while (y) {
         const int k = __builtin_ctzll(y);
         y &= y - 1;
-        atomicAdd(&acc[k], p);
+        fx_add(&acc[k], p);
       }
     }
   }
   s = block_sum_d(s, sh);
   __syncthreads();
   if (threadIdx.x == 0) rsq[r] = s;
-  if (threadIdx.x < L) P[r * L + threadIdx.x] = acc[threadIdx.x];
+  if (threadIdx.x < L) P[r * L + threadIdx.x] = fx_get(acc[threadIdx.x]);
 }
 
 // D[i][j] = sum_r w[r] n_i(rstr[r]) * (Y ? Y[r*L+j] : n_j(rstr[r]))  (one thread per (i,j))
@@ -281,10 +294,10 @@ __global__ void __launch_bounds__(RT) k_ab_rdm(int64_t nb, int L, const int32_t*
                                                const double* __restrict__ W, double* __restrict__ part) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* cb = reinterpret_cast<double*>(smem);
-  double* T = cb + chunk;
+  unsigned long long* T = reinterpret_cast<unsigned long long*>(cb + chunk);
   const int LL = L * L;
   const int w = blockIdx.x;
-  for (int x = threadIdx.x; x < LL; x += RT) T[x] = 0.0;
+  for (int x = threadIdx.x; x < LL; x += RT) T[x] = 0ull;
   for (int32_t e = wseg[w]; e < wseg[w + 1]; ++e) {
     const int32_t q = perm[e];
     const int64_t ia = rowof[q], ja = colSa[q];
@@ -303,14 +316,14 @@ __global__ void __launch_bounds__(RT) k_ab_rdm(int64_t nb, int L, const int32_t*
           const uint32_t p = pack[ee];
           const int jb = (int)(p & 0x3FFFFu) - (int)j0;
           const double v = (((p >> 30) & 1u) ? -wb : wb) * cb[jb];
-          atomicAdd(&T[(p >> 18) & 0xFFFu], v);
+          fx_add(&T[(p >> 18) & 0xFFFu], v);
         }
       }
     }
   }
   __syncthreads();
   (void)wkey;
-  for (int x = threadIdx.x; x < LL; x += RT) part[(int64_t)w * LL + x] = T[x];
+  for (int x = threadIdx.x; x < LL; x += RT) part[(int64_t)w * LL + x] = fx_get(T[x]);
 }
 
 // sum the chunk partials of each key (a,i) and write Gamma_ab[a,b,i,j], Gamma_ba[b,a,j,i]
@@ -437,16 +450,14 @@ void doubles_part(const Table& t, int L, int s, const double* X, int64_t ncols, 
 
 }  // namespace
 
-void rdm_compute(const RdmPlan& P, const double* W, double* one, double* two, hipStream_t st) {
+void rdm_compute(const RdmPlan& P, const double* W, double* WT, double* one, double* two, hipStream_t st) {
   const int L = P.L;
   const int64_t na = P.na, nb = P.nb, N = na * nb;
   const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
   SBDG_CHECK(hipMemsetAsync(one, 0, sizeof(double) * 2 * L2, st));
   SBDG_CHECK(hipMemsetAsync(two, 0, sizeof(double) * 4 * L4, st));
   if (N == 0) return;
-  DBuf<double> WT;
-  WT.alloc(N);
-  transpose_d(na, nb, W, WT.p, st);
+  transpose_d(na, nb, W, WT, st);
 
   // diagonal: density correlators
   {
@@ -461,7 +472,7 @@ void rdm_compute(const RdmPlan& P, const double* W, double* one, double* two, hi
     dens.alloc(2 * L);
     // rows of W: rsa[ia] = sum W^2, Pa[ia][j] = sum_ib W^2 n_j(ib)  (beta occupations)
     hipLaunchKernelGGL(k_rowocc, dim3((unsigned)na), dim3(RT), 0, st, nb, L, W, P.b, rsa.p, Pa.p);
-    hipLaunchKernelGGL(k_rowocc, dim3((unsigned)nb), dim3(RT), 0, st, na, L, WT.p, P.a, rsb.p, Pb.p);
+    hipLaunchKernelGGL(k_rowocc, dim3((unsigned)nb), dim3(RT), 0, st, na, L, WT, P.a, rsb.p, Pb.p);
     hipLaunchKernelGGL(k_corr, dim3(nb_for(L2, 64)), dim3(64), 0, st, na, L, P.a, rsa.p, nullptr, Daa.p);
     hipLaunchKernelGGL(k_corr, dim3(nb_for(L2, 64)), dim3(64), 0, st, nb, L, P.b, rsb.p, nullptr, Dbb.p);
     hipLaunchKernelGGL(k_corr, dim3(nb_for(L2, 64)), dim3(64), 0, st, na, L, P.a, nullptr, Pa.p, Dab.p);
@@ -483,8 +494,8 @@ void rdm_compute(const RdmPlan& P, const double* W, double* one, double* two, hi
   singles_part(*P.sa, L, 0, W, nb, P.a, P.b, one, two, st);
   doubles_part(*P.da, L, 0, W, nb, two, st);
   // beta pairs on the transposed vector (alpha spectators from the alpha strings)
-  singles_part(*P.sb, L, 1, WT.p, na, P.b, P.a, one, two, st);
-  doubles_part(*P.db, L, 1, WT.p, na, two, st);
+  singles_part(*P.sb, L, 1, WT, na, P.b, P.a, one, two, st);
+  doubles_part(*P.db, L, 1, WT, na, two, st);
 
   // alpha-beta
   const Table& sa = *P.sa;

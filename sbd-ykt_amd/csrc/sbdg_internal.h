@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,23 @@ struct Error {
   Error(int c, std::string m) : code(c), msg(std::move(m)) {}
 };
 
+// Device bytes held by the library's buffers (all devices of the process) and their
+// high-water mark since the last reset; sbdg_stats reports them.
+struct MemCount {
+  std::atomic<int64_t> now{0}, peak{0};
+  void add(int64_t b) {
+    const int64_t v = now.fetch_add(b) + b;
+    int64_t p = peak.load();
+    while (v > p && !peak.compare_exchange_weak(p, v)) {
+    }
+  }
+  void reset_peak() { peak.store(now.load()); }
+};
+inline MemCount& mem_count() {
+  static MemCount m;
+  return m;
+}
+
 // Device buffer owned by the context.
 template <class T>
 struct DBuf {
@@ -41,11 +59,17 @@ struct DBuf {
   void alloc(size_t count) {
     if (count == n && p) return;
     free();
-    n = count;
-    if (count) SBDG_CHECK(hipMalloc(&p, sizeof(T) * count));
+    if (count) {
+      SBDG_CHECK(hipMalloc(&p, sizeof(T) * count));
+      n = count;
+      mem_count().add((int64_t)(sizeof(T) * count));
+    }
   }
   void free() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      mem_count().add(-(int64_t)(sizeof(T) * n));
+    }
     p = nullptr;
     n = 0;
   }
@@ -201,9 +225,18 @@ struct SigmaPlan {
   hipStream_t st2 = nullptr;  // when set: the alpha-beta terms run there, beside the spin phases
   hipEvent_t fork = nullptr, join = nullptr;
 };
-// sigma = H c in the separated convention; scratch holds 3*na*nb elements of T.
+// sigma = H c in the separated convention.  Scratch: three vectors of na*nb elements of T
+// (c transposed, the beta product on it, the alpha-beta terms when they run on the second
+// stream), any device memory the caller does not use while sigma runs; none may overlap
+// c or sigma.
 template <class T>
-void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream_t st,
+struct SigmaScratch {
+  T* cT = nullptr;
+  T* sT = nullptr;
+  T* sab = nullptr;
+};
+template <class T>
+void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, const SigmaScratch<T>& scr, hipStream_t st,
                  hipEvent_t* ev = nullptr);
 size_t ab_lds_bytes(int64_t kbase, int64_t chunk);
 
@@ -264,6 +297,8 @@ void rotate_impl(T* X, int64_t ldx, int m, const double* U, int ld, int mk, int6
 void syev_device(const double* d_H, int m, int ld, double* d_E, double* d_U, hipStream_t st);
 void density(const double* W, const uint64_t* a, int64_t na, const uint64_t* b, int64_t nb, int L,
              double* d_dens, double* d_part, hipStream_t st);
+// d_rs[ia] = sum_ib W[ia,ib]^2 (fixed-order block sums)
+void row_sumsq(const double* W, int64_t na, int64_t nb, double* d_rs, hipStream_t st);
 
 // RDMs (rdm.hip)
 struct RdmPlan {
@@ -278,8 +313,9 @@ struct RdmPlan {
   const AbTable* ab = nullptr;
 };
 // one[2][L^2], two[4][L^4] in the reference layout (correlation.h:194-444) from the
-// separated-convention vector W.
-void rdm_compute(const RdmPlan& P, const double* W, double* d_one, double* d_two, hipStream_t st);
+// separated-convention vector W; WT: scratch of na*nb doubles (W transposed).  Bitwise
+// reproducible: every sum is taken in a fixed order or as exact integer (fixed-point) adds.
+void rdm_compute(const RdmPlan& P, const double* W, double* WT, double* d_one, double* d_two, hipStream_t st);
 // Hellmann-Feynman gradient: dh[p*L+q] = g*(one_a+one_b)[p+L*q];
 // deri[((p*L+q)*L+r)*L+s] = g/2 * sum_blk two[blk][p + L*r + L^2*q + L^3*s]
 void rdm_backward(const double* d_one, const double* d_two, int L, double g, double* d_dh,

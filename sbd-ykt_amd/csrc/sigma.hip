@@ -656,10 +656,10 @@ size_t ab_lds_bytes(int64_t kbase, int64_t chunk) {
 }
 
 template <class T>
-void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream_t st, hipEvent_t* ev) {
-  const int64_t N = P.na * P.nb;
-  T* cT = scratch;
-  T* sT = scratch + N;
+void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, const SigmaScratch<T>& scr, hipStream_t st,
+                 hipEvent_t* ev) {
+  T* cT = scr.cT;
+  T* sT = scr.sT;
   const T* hii;
   const T* eri;
   if constexpr (sizeof(T) == 8) {
@@ -672,7 +672,7 @@ void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream
   if (ev) SBDG_CHECK(hipEventRecord(ev[0], st));
   // the alpha-beta terms (LDS-bound) into their own buffer on the second stream, while the
   // spin phases (HBM-bound) run on this one; the transpose-add joins them
-  T* sab = scratch + 2 * N;
+  T* sab = scr.sab;
   if (P.st2) {
     SBDG_CHECK(hipEventRecord(P.fork, st));
     SBDG_CHECK(hipStreamWaitEvent(P.st2, P.fork, 0));
@@ -711,7 +711,9 @@ void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, T* scratch, hipStream
   SBDG_CHECK(hipGetLastError());
 }
 
-template void sigma_apply<double>(const SigmaPlan&, const double*, double*, double*, hipStream_t, hipEvent_t*);
-template void sigma_apply<float>(const SigmaPlan&, const float*, float*, float*, hipStream_t, hipEvent_t*);
+template void sigma_apply<double>(const SigmaPlan&, const double*, double*, const SigmaScratch<double>&,
+                                  hipStream_t, hipEvent_t*);
+template void sigma_apply<float>(const SigmaPlan&, const float*, float*, const SigmaScratch<float>&, hipStream_t,
+                                 hipEvent_t*);
 
 }  // namespace sbdg

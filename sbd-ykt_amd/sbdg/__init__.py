@@ -47,6 +47,7 @@ class Stats(C.Structure):
         ("t_tables", C.c_double), ("t_davidson", C.c_double), ("t_sigma", C.c_double),
         ("t_final", C.c_double), ("t_rdm", C.c_double), ("terms", C.c_int64), ("nnz", C.c_int64 * 4),
         ("core_members", C.c_int64 * 2), ("core_direct", C.c_int64 * 2),
+        ("mem_now", C.c_int64), ("mem_peak", C.c_int64),
     ]
 
 
@@ -86,6 +87,7 @@ SIGNATURES = {
     "sbdg_get_hii": (C.c_int, [_P, _D]),
     "sbdg_sigma": (C.c_int, [_P, _D, _D, C.c_int]),
     "sbdg_bench_sigma": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "sbdg_memory_plan": (C.c_int, [C.POINTER(Options), C.c_int, C.c_int64, C.c_int64, _I64, _I64, _I64, _I64]),
 }
 
 _lib = None
@@ -287,6 +289,19 @@ class Solver:
         return ms
 
 
+def memory_plan(norb, na, nb, nnz, core_members, core_direct, **options):
+    """Planned device bytes (no device needed): dict of the build's resident tables, the
+    Davidson vectors for these options and the RDMs' transient peak (sbdg_memory_plan)."""
+    o = default_options(**options)
+    a = lambda v: (C.c_int64 * len(v))(*[int(x) for x in v])
+    out = (C.c_int64 * 3)()
+    rc = lib().sbdg_memory_plan(C.byref(o), int(norb), int(na), int(nb), a(nnz), a(core_members), a(core_direct),
+                                out)
+    if rc != OK:
+        raise SbdgError(rc, "bad memory plan arguments")
+    return dict(build=out[0], davidson=out[1], rdm=out[2], total=out[0] + out[1] + out[2])
+
+
 @dataclass
 class DiagResult:
     energy: float
@@ -322,24 +337,27 @@ def _torch_function():
     class GroundStateEnergy(torch.autograd.Function):
         """E(h1, eri) = lowest eigenvalue in the selected basis.
 
-        forward: Davidson on the GPU (FP64), then the RDMs of the converged vector.
-        backward: dE/dh1[p,q] = gamma[p + L q] (spin-summed),
-                  dE/deri[p,q,r,s] = 1/2 sum_blk Gamma[blk][p + L r + L^2 q + L^3 s],
-        contracted on the device (Hellmann-Feynman; the Davidson is not re-run)."""
+        forward: Davidson on the GPU (FP64), then the RDMs of the converged vector and their
+                 contraction into dE/dh1[p,q] = gamma[p + L q] (spin-summed) and
+                 dE/deri[p,q,r,s] = 1/2 sum_blk Gamma[blk][p + L r + L^2 q + L^3 s] on the device
+                 (Hellmann-Feynman), kept with this node of the graph;
+        backward: those derivatives times the incoming gradient (the Davidson is not re-run).
+        The derivatives belong to this forward's solve: later solves on the same solver (a
+        second energy in the graph, a line search) do not change them."""
 
         @staticmethod
         def forward(ctx, h1, eri, solver: Solver, core: float = 0.0):
             solver.set_integrals(core, h1.detach().cpu().double().numpy(), eri.detach().cpu().double().numpy())
             e = solver.run()
             solver.rdm(compute=True)
-            ctx.solver = solver
-            ctx.dev = h1.device
+            dh, de = solver.rdm_backward(1.0)
+            ctx.dh = torch.from_numpy(dh).to(h1.device)
+            ctx.de = torch.from_numpy(de).to(h1.device)
             return torch.tensor(e, dtype=torch.float64, device=h1.device)
 
         @staticmethod
         def backward(ctx, g):
-            dh, de = ctx.solver.rdm_backward(float(g.item()))
-            return (torch.from_numpy(dh).to(ctx.dev), torch.from_numpy(de).to(ctx.dev), None, None)
+            return g * ctx.dh, g * ctx.de, None, None
 
     return GroundStateEnergy
 

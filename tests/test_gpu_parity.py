@@ -498,6 +498,30 @@ def test_autograd_function():
     assert abs(ident - float(e)) < 1e-9
 
 
+def test_autograd_two_forwards_one_solver():
+    """Two energies from one solver before any backward: each backward uses the RDMs of its
+    own forward (the second solve, at other integrals, does not leak into the first)."""
+    import torch
+    case = workload_small(8, 3, 3, 40, 40, 61)
+    ref = solver(case, max_it=100, max_nb=12, eps=1e-11)
+    h1 = torch.tensor(case["h1"], requires_grad=True)
+    eri = torch.tensor(case["eri"], requires_grad=True)
+    sbdg.ground_state_energy(h1, eri, ref, case["core"]).backward()
+    g_h, g_e = h1.grad.clone(), eri.grad.clone()
+    h1.grad = eri.grad = None
+    s = solver(case, max_it=100, max_nb=12, eps=1e-11)
+    e1 = sbdg.ground_state_energy(h1, eri, s, case["core"])
+    e2 = sbdg.ground_state_energy(1.1 * h1, eri, s, case["core"])
+    assert abs(float(e1) - float(e2)) > 1e-3
+    e1.backward()
+    assert torch.equal(h1.grad, g_h) and torch.equal(eri.grad, g_e)
+    h1.grad = eri.grad = None
+    (2.0 * e2).backward()
+    h1b = torch.tensor(1.1 * case["h1"], requires_grad=True)
+    sbdg.ground_state_energy(h1b, eri.detach(), solver(case, max_it=100, max_nb=12, eps=1e-11), case["core"]).backward()
+    assert torch.allclose(h1.grad, 2.2 * h1b.grad, rtol=1e-12, atol=1e-14)  # the second point, chain rule included
+
+
 def test_carryover_matches_row_weights():
     case = medium_case()
     s = solver(case, max_it=100, max_nb=10, eps=1e-8, threshold=0.01)

@@ -127,3 +127,30 @@ def test_bench_measures_one_gpu_only():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], cwd=ROOT,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+HBM_BYTES = 288e9  # MI355X
+
+
+def test_c4_mixed_davidson_fits_hbm():
+    """The largest workload (norb=58, 12+12 electrons, 40,000 x 40,000 strings, 1.6e9
+    determinants) with the mixed Davidson at its settings (subspace 8, restart to 3): the
+    library's planned device bytes (tables from the oracle's excitation counts; every
+    string's C(12,2) electron pairs taken as core memberships and every double as a directly
+    read pair, both upper bounds) stay below the 288 GB of HBM.  The FP64 basis alone is
+    8 x 12.8 GB; the plan also shows what the FP64 Davidson would need."""
+    wl = syn.config_workload("c4_L58_12e12e_40000")
+    p = oracle.Problem(wl.norb, wl.core, wl.h1, wl.astr, wl.bstr, eri=wl.eri)
+    nnz = [oracle.lib().orc_table_nnz(p._p, w) for w in range(4)]
+    na, nb, N = len(wl.astr), len(wl.bstr), wl.ndet
+    mem = [na * 66, nb * 66]
+    plan = sbdg.memory_plan(wl.norb, na, nb, nnz, mem, [nnz[1], nnz[3]], precision=1, max_nb=8, restart_keep=3)
+    # the Davidson vectors: 8 FP64 basis + 8 FP32 sigma vectors, W, R, T, HX (FP64), c32 (FP32)
+    assert plan["davidson"] == N * (8 * 8 + 8 * 4 + 4 * 8 + 4)
+    assert plan["total"] < HBM_BYTES, plan
+    # the build: at least the diagonal (FP64 + FP32) and the 16 GiB core workspace
+    assert plan["build"] > 12 * N + 16 * 2 ** 30
+    fp64 = sbdg.memory_plan(wl.norb, na, nb, nnz, mem, [nnz[1], nnz[3]], precision=0, max_nb=8)
+    assert fp64["davidson"] == N * (8 * 8 * 2 + 3 * 8)
+    print(f"c4 plan: mixed {plan['total'] / 1e9:.1f} GB (build {plan['build'] / 1e9:.1f}, Davidson "
+          f"{plan['davidson'] / 1e9:.1f}, RDM {plan['rdm'] / 1e9:.1f}); FP64 {fp64['total'] / 1e9:.1f} GB")

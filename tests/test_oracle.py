@@ -51,6 +51,26 @@ def test_hii_and_sigma_vs_reference(name):
 
 
 @pytest.mark.parametrize("name", SMALL_CASES)
+def test_sigma_rows_vs_sigma_and_reference(name):
+    """The row-restricted sigma (the checker of the full-size sigma tests and the CPU
+    baseline's workload) against the whole-vector sigma and the reference's own sigma, on
+    every row block of a split, with the diagonal passed in and evaluated in place."""
+    g, case = small_case(name)
+    p = problem(case)
+    vec = np.random.default_rng(int(g["vec_seed"])).normal(size=p.ndet)
+    full = p.sigma(vec, hii=g["hii"])
+    nb = p.nb
+    cuts = sorted({0, 1, p.na // 3, p.na - 1, p.na})
+    for ia0, ia1 in zip(cuts[:-1], cuts[1:]):
+        blk = slice(ia0 * nb, ia1 * nb)
+        for hii in (g["hii"], None):
+            rows = p.sigma_rows(vec, hii, ia0, ia1)
+            assert np.abs(rows[blk] - full[blk]).max() <= 1e-13 * np.abs(full).max()
+            assert np.abs(rows[blk] - g["sigma"][blk]).max() <= 1e-12 * np.abs(g["sigma"]).max()
+            assert not rows[:ia0 * nb].any() and not rows[ia1 * nb:].any()  # nothing outside the rows
+
+
+@pytest.mark.parametrize("name", SMALL_CASES)
 def test_davidson_energy_rdm_vs_reference(name):
     g, case = small_case(name)
     p = problem(case)
