@@ -515,6 +515,10 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
         }
       }
     }
+    // a column's owner thread changes from chunk to chunk: the next chunk's read-modify-write
+    // of an element must see this chunk's store.  The barriers inside the row loop order
+    // them only when the bra row has alpha singles; this one also covers rows without.
+    if (nch > 1) __syncthreads();
   }
 }
 

@@ -253,3 +253,36 @@ def test_rdm_bitwise_reproducible():
     one_o, two_o = p.rdm(s.vector())
     rdm_close(one, two, one_o, two_o, 1e-9, mixed_sum=True)
     s.close()
+
+
+# ---------------------------------------------------------------- repeated calls, reused memory
+
+def test_sigma_repeated_calls_several_ket_chunks():
+    """Regression: 120 x 8,000 sampled strings at norb=36 (the alpha-beta kernel takes the
+    ket columns in two chunks; a third of the alpha rows have no alpha single).  sigma on one
+    context several times in a row, with the alpha-beta kernel beside the spin phases (its
+    terms in a scratch vector that holds the previous call's values) and in sequence, each
+    time equal to the oracle; then an FP64 Davidson and, on the same context (its vectors now
+    holding FP64 data), the mixed one: the energies agree, and the FP64 energy equals the
+    oracle's Rayleigh quotient of the returned vector."""
+    case = workload_small(36, 9, 9, 120, 8000, 29)
+    s, p = solver(case), oproblem(case)
+    rp, _, _, _ = s.table(0)
+    assert np.any(np.diff(rp) == 0)  # alpha rows without alpha singles
+    rng = np.random.default_rng(3)
+    xs = [rng.normal(size=s.ndet) for _ in range(2)]
+    os_ = [p.sigma(x) for x in xs]
+    for overlap in (1, 0, 1):
+        s.set_options(overlap=overlap)
+        for x, o in zip(xs, os_):
+            assert np.abs(s.sigma(x) - o).max() <= 1e-12 * np.abs(o).max(), overlap
+            assert np.abs(s.sigma(x, precision=1) - o).max() <= 2e-5 * np.abs(o).max(), overlap
+    s.set_options(overlap=1, max_it=100, max_nb=10, restart_keep=3, eps=1e-7, precision=0)
+    e64 = s.run()
+    w = s.vector()
+    e_rq = float(w @ p.sigma(w)) / float(w @ w)
+    assert abs(e64 - e_rq) < 1e-10, (e64, e_rq)
+    s.set_options(precision=1, refresh_k=4, eps=1e-6)
+    em = s.run()
+    assert s.stats()["sigma_fp32"] > 0 and abs(em - e64) < 1e-6, (em, e64)
+    s.close()
