@@ -195,6 +195,16 @@ int sbdg_get_hii(sbdg_ctx *ctx, double *hii);
 /* sigma = H c for a host vector in the reference convention (mult, tpb/mult.h:376-632).
  * precision 0: FP64, 1: FP32 vectors/integrals (result widened to FP64). */
 int sbdg_sigma(sbdg_ctx *ctx, const double *c, double *sigma, int precision);
+/* sbdg_sigma on device vectors, allocating nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  x and y are distinct device buffers
+ * of na*nb doubles owned by the caller (reference phases, as sbdg_sigma); the context must
+ * be built (sbdg_build), else SBDG_ERR_STATE.  Its scratch is a static device array of
+ * 256 MiB per device, held until the process ends: at most 8,388,608 determinants in FP64
+ * (4 vectors), 13,421,772 in FP32 (5 FP32 vectors); larger problems get SBDG_ERR_LIMIT.
+ * Every context on a device shares that device's array: calls that overlap on one device
+ * (from several host threads) race.  Returns after the result is in y. */
+int sbdg_sigma_device(sbdg_ctx *ctx, const double *x, double *y, int precision);
 /* Times `reps` sigma builds on resident device vectors with HIP events on the context
  * stream.  ms[0] total per build; ms[1..5] per kernel phase (transpose, beta product,
  * alpha product + diagonal, transpose-add, alpha-beta) when options.overlap is 0.  With

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <random>
 #include <string>
 #include <unordered_map>
@@ -40,6 +41,40 @@ using namespace sbdg;
 namespace {
 constexpr int MAXSUB = 48;
 constexpr size_t LDS_BUDGET = 160 * 1024;  // alpha-beta kernel: all of a CU's LDS, one 1024-thread workgroup
+
+// Scratch of sbdg_sigma_device, the sigma entry point that allocates nothing: one array
+// per device, held from the library's load until the process ends.  On the host the
+// array's name stands for no device address: sbdg_create resolves each device's copy once
+// (hipGetSymbolAddress, with that device current) and every use goes through the resolved
+// address.  FP64 sigma uses 4 vectors of na*nb doubles of it, FP32 5 vectors of floats.
+constexpr size_t SCRATCH_BYTES = (size_t)256 << 20;
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[SCRATCH_BYTES / sizeof(double)];
+
+struct ScratchTable {
+  std::mutex mu;
+  void* ptr[SCRATCH_DEVICES] = {};
+  hipError_t rc[SCRATCH_DEVICES] = {};
+};
+ScratchTable& scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+
+// The device address of `scratch` on `device` (which must be current), resolved on the
+// first call for that device; nullptr when the runtime cannot resolve it.
+void* scratch_address(int device) {
+  if (device < 0 || device >= SCRATCH_DEVICES) return nullptr;
+  ScratchTable& t = scratch_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  if (!t.ptr[device] && t.rc[device] == hipSuccess) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) t.ptr[device] = p;
+    else t.rc[device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+  }
+  return t.ptr[device];
+}
 }  // namespace
 
 struct sbdg_ctx {
@@ -80,6 +115,7 @@ struct sbdg_ctx {
   sbdg_stats stats{};
   std::vector<double> traj_t, traj_e;
   double energy = 0.0;
+  void* scratch = nullptr;  // this device's copy of the static scratch (sbdg_sigma_device)
 };
 
 namespace {
@@ -835,6 +871,7 @@ sbdg_ctx* sbdg_create(int device) {
     return nullptr;
   }
   sbdg_default_options(&c->opt);
+  c->scratch = scratch_address(device);  // once per device; checked where it is used
   return c;
 }
 
@@ -1308,6 +1345,38 @@ int sbdg_sigma(sbdg_ctx* c, const double* x, double* y, int precision) {
     }
     sign_transform(c->d_a.p, na, c->d_b.p, nb, out.p, out.p, c->st);
     SBDG_CHECK(hipMemcpyAsync(y, out.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+    SBDG_CHECK(hipStreamSynchronize(c->st));
+    return SBDG_OK;
+  });
+}
+
+int sbdg_sigma_device(sbdg_ctx* c, const double* x, double* y, int precision) {
+  return guard(c, [&]() {
+    if (!x || !y) return fail(c, SBDG_ERR_ARG, "null vector");
+    if (x == y) return fail(c, SBDG_ERR_ARG, "input and output vectors must differ");
+    if (precision != 0 && precision != 1) return fail(c, SBDG_ERR_ARG, "precision must be 0 or 1");
+    if (!c->built) return fail(c, SBDG_ERR_STATE, "sbdg_sigma_device needs a built context (sbdg_build)");
+    if (!c->scratch) return fail(c, SBDG_ERR_HIP, "the device scratch array has no address on this device");
+    const int64_t na = c->a.size(), nb = c->b.size(), N = na * nb;
+    const size_t need = precision == 0 ? (size_t)N * 4 * sizeof(double) : (size_t)N * 5 * sizeof(float);
+    if (need > SCRATCH_BYTES)
+      return fail(c, SBDG_ERR_LIMIT,
+                  "sbdg_sigma_device: " + std::to_string(N) + " determinants need " + std::to_string(need) +
+                      " scratch bytes, beyond its " + std::to_string(SCRATCH_BYTES) +
+                      " (at most " + std::to_string(SCRATCH_BYTES / 32) + " determinants in FP64, " +
+                      std::to_string(SCRATCH_BYTES / 20) + " in FP32)");
+    if (precision == 0) {
+      double* S = static_cast<double*>(c->scratch);
+      sign_transform(c->d_a.p, na, c->d_b.p, nb, x, S, c->st);  // separated phases
+      sigma64(c, S, y, SigmaScratch<double>{S + N, S + 2 * N, S + 3 * N});
+    } else {
+      float* F = static_cast<float*>(c->scratch);
+      sign_transform(c->d_a.p, na, c->d_b.p, nb, x, y, c->st);  // y: staging of the FP64 input
+      cast_d2f(y, F, N, c->st);
+      sigma32(c, F, F + N, SigmaScratch<float>{F + 2 * N, F + 3 * N, F + 4 * N});
+      cast_f2d(F + N, y, N, c->st);
+    }
+    sign_transform(c->d_a.p, na, c->d_b.p, nb, y, y, c->st);  // back to the reference phases
     SBDG_CHECK(hipStreamSynchronize(c->st));
     return SBDG_OK;
   });

@@ -86,6 +86,7 @@ SIGNATURES = {
     "sbdg_get_table": (C.c_int, [_P, C.c_int, _I64, _I32, _U32, _D]),
     "sbdg_get_hii": (C.c_int, [_P, _D]),
     "sbdg_sigma": (C.c_int, [_P, _D, _D, C.c_int]),
+    "sbdg_sigma_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int]),
     "sbdg_bench_sigma": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "sbdg_memory_plan": (C.c_int, [C.POINTER(Options), C.c_int, C.c_int64, C.c_int64, _I64, _I64, _I64, _I64]),
 }
@@ -269,6 +270,12 @@ class Solver:
         out = np.zeros(self.ndet)
         self._chk(lib().sbdg_sigma(self._c, _d(c), _d(out), int(precision)))
         return out
+
+    def sigma_device(self, x_ptr, y_ptr, precision=0):
+        """sigma on caller-owned device buffers (addresses of na*nb doubles), allocating
+        nothing: sbdg_sigma_device."""
+        self._chk(lib().sbdg_sigma_device(self._c, C.c_void_p(int(x_ptr)), C.c_void_p(int(y_ptr)),
+                                          int(precision)))
 
     def save_wavefunction(self, prefix, bit_length=20):
         """diag's savename: writes <prefix>000000 in the reference's restart layout."""
