@@ -105,7 +105,9 @@ struct sbdg_ctx {
   DBuf<float> Wp32;
   DBuf<double> Zbuf;      // core-product workspace
   AbTable ab64;     // beta singles by chunk (RDMs)
-  EllTable ell;      // beta singles, ELL (alpha-beta sigma, FP64 and FP32 alike)
+  EllTable ell;      // beta singles, ELL (alpha-beta sigma, FP64; FP32 too when the geometries agree)
+  EllTable ell32;    // the FP32 ELL when its launch geometry differs (wide rows)
+  bool own_ell32 = false;
   // solver state
   DBuf<double> V, HV, W, R, T, HX, scr64, small;
   DBuf<float> HV32, c32, s32, scr32;
@@ -163,8 +165,7 @@ SigmaPlan plan(sbdg_ctx* c, bool fp32) {
   P.ma = &c->Ma;
   P.mb = &c->Mb;
   P.sa = &c->Sa;
-  P.ell = &c->ell;
-  (void)fp32;
+  P.ell = (fp32 && c->own_ell32) ? &c->ell32 : &c->ell;
   P.eri = c->d_eri.p;
   P.eri32 = c->d_eri32.p;
   P.hii = c->d_hii.p;
@@ -253,7 +254,15 @@ void build(sbdg_ctx* c) {
   }
   build_ab(c->Sb, L, std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - (int64_t)L * L)),
            c->ab64, c->st);
-  build_ell(c->Sb, L, choose_chunk(nb, L), 16, c->ell, c->st);  // both precisions
+  build_ell(c->Sb, L, choose_chunk(nb, L), 8, c->ell, c->st);
+  {
+    int nt64, m64, nt32, m32;
+    ab_geometry(nb, 8, nt64, m64);
+    ab_geometry(nb, 4, nt32, m32);
+    c->own_ell32 = nt64 != nt32 || m64 != m32;
+    if (c->own_ell32) build_ell(c->Sb, L, choose_chunk(nb, L), 4, c->ell32, c->st);
+    else c->ell32.release();
+  }
   c->d_hii.alloc(std::max<int64_t>(N, 1));
   c->d_hii32.alloc(std::max<int64_t>(N, 1));
   compute_hii(c->d_a.p, na, c->d_b.p, nb, L, c->core, c->d_h1.p, c->d_eri.p, c->d_hii.p, c->st);
@@ -344,7 +353,11 @@ int64_t build_bytes(const sbdg_options& o, int L, int64_t na, int64_t nb, const 
   const int64_t nch = (nb + std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32) - 1) /
                       std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32);
   const int64_t groups = (nb + 63) / 64 + 1024;  // tiles round the lane positions up
-  b += 4 * (64 * Sb + 192 * groups * nch) + 4 * nch * groups * 64 + 16 * groups * nch;
+  int nt64, m64, nt32, m32;
+  ab_geometry(nb, 8, nt64, m64);
+  ab_geometry(nb, 4, nt32, m32);
+  const int nell = (nt64 != nt32 || m64 != m32) ? 2 : 1;  // an FP32 ELL of its own (wide rows)
+  b += nell * (4 * (64 * Sb + 192 * groups * nch) + 4 * nch * groups * 64 + 16 * groups * nch);
   return b;
 }
 

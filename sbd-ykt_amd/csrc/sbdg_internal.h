@@ -145,6 +145,13 @@ struct EllTable {
   DBuf<int32_t> nq;         // ntiles*(nt/64)*nch*m: four-row batches of each group of the stream
   DBuf<uint32_t> ent;
   DBuf<int32_t> colperm;    // nch * groups*64: column of each lane position in each chunk (-1: none)
+  void release() {
+    soff.free();
+    nq.free();
+    ent.free();
+    colperm.free();
+    nch = nt = m = 0;
+  }
 };
 
 // Double excitations of one spin grouped by their (n-2)-electron core: a double pair
@@ -193,7 +200,9 @@ void build_occ_table(const uint64_t* d_str, int64_t n, int L, const double* d_er
 // W[P*npair + Q] (FP64 and FP32), npair = L(L-1)/2
 void build_pair_integrals(const double* d_eri, int L, double* W, float* W32, hipStream_t st);
 void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st);
-void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st);
+// prec_bytes: 8 (FP64) or 4 (FP32) -- the launch geometry depends on the accumulator size
+void build_ell(const Table& sb, int L, int64_t chunk, int prec_bytes, EllTable& e, hipStream_t st);
+void ab_geometry(int64_t nb, int prec_bytes, int& nt, int& m);
 void compute_hii(const uint64_t* d_a, int64_t na, const uint64_t* d_b, int64_t nb, int L, double core,
                  const double* d_h1, const double* d_eri, double* d_hii, hipStream_t st);
 

@@ -385,8 +385,12 @@ __device__ inline void ab_batch(const uint32_t (&p)[4], T (&s)[AbRows<T>::G]) {
 // rows per batch); the next batch is loaded while the current one is used, across group
 // boundaries, and after the last batch the stream's first batch is loaded again for the
 // next ket rows.
-template <class T, int M>
-__global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
+// Workgroup w runs on XCD w % 8: consecutive workgroups of one XCD take the column tiles of
+// one bra row (rows wider than one tile), so the tiles' staged ket rows and slices meet in
+// that XCD's L2.  NTMAX bounds the workgroup (1024 threads; 768 for the widest FP64 rows,
+// whose accumulators need more than 128 VGPRs).
+template <class T, int M, int NTMAX>
+__global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int64_t ntiles,
                                              const int64_t* __restrict__ rpSa,
                                              const int32_t* __restrict__ colSa,
                                              const uint32_t* __restrict__ excSa,
@@ -406,8 +410,10 @@ __global__ void __launch_bounds__(1024) k_ab(int64_t nb, int L, int64_t ntiles,
   const int64_t LL = (int64_t)L * L;
   const int NT = blockDim.x;
   const int64_t npos = groups * 64;
-  const int64_t ia = blockIdx.x / ntiles;
-  const int64_t tile = blockIdx.x - ia * ntiles;
+  const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int64_t tile = slot % ntiles;
+  const int64_t ia = (slot / ntiles) * 8 + xcd;
+  if (ia >= na) return;  // the grid rounds the rows up to a multiple of 8 (whole workgroup)
   const int64_t col0 = tile * (int64_t)NT * M;
   const int lane = threadIdx.x & 63;
   const int64_t s0 = (tile * (NT >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * nch;  // wave-uniform
@@ -613,18 +619,19 @@ void launch_transpose(int64_t R, int64_t C, const T* in, T* out, hipStream_t st,
   hipLaunchKernelGGL((k_transpose<T, ADD>), g, dim3(256), 0, st, R, C, in, out, add);
 }
 
-template <class T, int M>
+template <class T, int M, int NTMAX = 1024>
 void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* sigma, hipStream_t st, int assign) {
   const EllTable& E = *P.ell;
-  const int64_t nblk = E.ntiles * P.na;
+  if (E.nt > NTMAX || E.nt % 64) throw Error(SBDG_ERR_STATE, "alpha-beta workgroup size");
+  const int64_t nblk = (P.na + 7) / 8 * 8 * E.ntiles;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
   static const bool lds_at_zero = [] {  // entries hold absolute LDS byte addresses
     hipFuncAttributes fa;
-    return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_ab<T, M>)) == hipSuccess &&
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_ab<T, M, NTMAX>)) == hipSuccess &&
            fa.sharedSizeBytes == 0;
   }();
   if (!lds_at_zero) throw Error(SBDG_ERR_STATE, "alpha-beta kernel has static LDS: entry addresses invalid");
-  hipLaunchKernelGGL((k_ab<T, M>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.nb, P.L, E.ntiles,
+  hipLaunchKernelGGL((k_ab<T, M, NTMAX>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.na, P.nb, P.L, E.ntiles,
                      P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.soff.p, E.nq.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
                      E.sb2, E.kbase, E.groups, eri, c, sigma, assign);
 }
@@ -644,7 +651,21 @@ void launch_ab(const SigmaPlan& P, const T* eri, const T* c, T* sigma, hipStream
     case 16: launch_ab_m<T, 16>(P, lds, eri, c, sigma, st, assign); break;
     case 20: launch_ab_m<T, 20>(P, lds, eri, c, sigma, st, assign); break;
     case 24: launch_ab_m<T, 24>(P, lds, eri, c, sigma, st, assign); break;
-    default: throw Error(SBDG_ERR_STATE, "alpha-beta columns per thread");
+    default:
+      if constexpr (sizeof(T) == 8) {  // 768 threads (ab_geometry)
+        switch (E.m) {
+          case 32: launch_ab_m<T, 32, 768>(P, lds, eri, c, sigma, st, assign); return;
+          case 40: launch_ab_m<T, 40, 768>(P, lds, eri, c, sigma, st, assign); return;
+          case 48: launch_ab_m<T, 48, 768>(P, lds, eri, c, sigma, st, assign); return;
+          case 53: launch_ab_m<T, 53, 768>(P, lds, eri, c, sigma, st, assign); return;
+        }
+      } else {
+        switch (E.m) {
+          case 40: launch_ab_m<T, 40>(P, lds, eri, c, sigma, st, assign); return;
+          case 48: launch_ab_m<T, 48>(P, lds, eri, c, sigma, st, assign); return;
+        }
+      }
+      throw Error(SBDG_ERR_STATE, "alpha-beta columns per thread");
   }
 }
 

@@ -476,15 +476,29 @@ void build_ab(const Table& sb, int L, int64_t chunk, AbTable& ab, hipStream_t st
 }
 
 // Alpha-beta launch geometry: nt threads, m columns per thread, so that one workgroup
-// covers a whole row when m <= 24 (the ket row is then staged once per pair).
-static void ab_geometry(int64_t nb, int& nt, int& m) {
+// covers a whole row (the ket rows and integral slices are then staged once per bra row,
+// not once per column tile).  The FP64 accumulators bound m: 1024 threads (four waves per
+// SIMD, 128 VGPRs) hold up to 24 columns each; beyond 24,576 columns FP64 takes 768 threads
+// (three waves per SIMD, 168 VGPRs) with up to 53 columns each.  FP32 holds up to 48
+// columns at 1024 threads.  Wider rows take several column tiles at the largest m.
+void ab_geometry(int64_t nb, int elem, int& nt, int& m) {
   nt = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (nb + 63) / 64 * 64));
-  const int64_t need = (nb + nt - 1) / nt;
-  static const int ms[] = {1, 2, 4, 8, 12, 16, 20, 24};
-  m = 24;
-  for (int x : ms)
-    if (need <= x) {
-      m = x;
+  static const int m1024_64[] = {1, 2, 4, 8, 12, 16, 20, 24};
+  static const int m768_64[] = {32, 40, 48, 53};
+  static const int m1024_32[] = {1, 2, 4, 8, 12, 16, 20, 24, 40, 48};
+  const int* ms = elem == 4 ? m1024_32 : m1024_64;
+  int nms = elem == 4 ? 10 : 8;
+  int64_t need = (nb + nt - 1) / nt;
+  if (elem != 4 && need > 24) {
+    nt = 768;
+    need = (nb + nt - 1) / nt;
+    ms = m768_64;
+    nms = 4;
+  }
+  m = ms[nms - 1];
+  for (int i = 0; i < nms; ++i)
+    if (need <= ms[i]) {
+      m = ms[i];
       break;
     }
 }
@@ -566,7 +580,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
   }
 }
 
-void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hipStream_t st) {
+void build_ell(const Table& sb, int L, int64_t chunk, int prec_bytes, EllTable& e, hipStream_t st) {
   const int64_t nb = sb.n;
   if (L > 64) throw Error(SBDG_ERR_LIMIT, "norb beyond 64");
   e.sb2 = ab_second_slice(L);
@@ -574,7 +588,7 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
   if (16 * (e.kbase + chunk + 1) > (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta LDS beyond 256 KiB");
   e.chunk = chunk;
   e.nch = (int)std::max<int64_t>(1, (nb + chunk - 1) / chunk);
-  ab_geometry(nb, e.nt, e.m);
+  ab_geometry(nb, prec_bytes, e.nt, e.m);
   const int64_t tw = (int64_t)e.nt * e.m;
   e.ntiles = std::max<int64_t>(1, (nb + tw - 1) / tw);
   const int64_t npos = e.ntiles * tw;
@@ -667,7 +681,7 @@ void build_ell(const Table& sb, int L, int64_t chunk, int elem, EllTable& e, hip
   // pass groups of the kernel's 16-byte LDS reads, as measured on gfx950 (the alpha-beta
   // kernel at norb=36, 20,000^2 with four candidate models; this one was fastest): 32 lanes
   // over 16 slots of 16 bytes (256 bytes of banks)
-  const int P = elem >= 16 ? 32 : 16, S = 16;
+  const int P = 32, S = 16;
   const uint32_t kbase = (uint32_t)e.kbase;  // ket vectors follow the slice vectors
   std::vector<std::vector<uint32_t>> lists(64);
   for (int64_t g = 0; g < e.groups; ++g)
