@@ -194,11 +194,9 @@ def fp64_subspace(s, st, na, nb, L, want, margin=4e9):
     of them) fit the device's free memory plus what the library holds beyond its tables (the
     Davidson releases the benchmark vectors and the other precision's vectors first): the FP64
     reference energy of the largest workload needs one."""
-    import torch
-
     import sbdg
 
-    free, _ = torch.cuda.mem_get_info(0)
+    free, _ = sbdg.device_memory()
     free += s.stats()["mem_now"] - st["mem_now"]  # st: right after the build
     for m in range(want, 2, -1):
         need = sbdg.memory_plan(L, na, nb, st["nnz"], st["core_members"], st["core_direct"],

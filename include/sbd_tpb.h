@@ -6,6 +6,7 @@
 //   sbd::tpb::diag                             (chemistry/tpb/sbdiag.h:113-124)
 //   sbd::LoadAlphaDets, sbd::makestring       (makedeterminants.h:160-185,
 //                                               bit_manipulation.h:135-150)
+//   sbd::ShuffleDet                            (determinants.h:589-594)
 // so a caller of the reference switches by including this header instead of sbd/sbd.h
 // and linking libsbdg.so.  Differences:
 //   * the communicator argument is accepted (any type) and ignored: one process, one GPU;
@@ -28,6 +29,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <map>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -113,6 +115,16 @@ inline void LoadAlphaDets(const std::string& file, std::vector<std::vector<size_
   };
   std::sort(adet.begin(), adet.end(), less);
   adet.erase(std::unique(adet.begin(), adet.end()), adet.end());
+}
+
+// ShuffleDet (determinants.h:589-594): the strings after the first (the Hartree-Fock one
+// stays in front) in the order std::shuffle gives with a std::mt19937 seeded by `seed`;
+// the sample driver applies it with seeds 1729 (alpha) and 137 (beta) under --shuffle
+// (samples/selected_basis_diagonalization/main.cc:123-128).
+inline void ShuffleDet(std::vector<std::vector<size_t>>& det, unsigned int seed) {
+  if (det.size() <= 1) return;
+  std::mt19937 g(seed);
+  std::shuffle(det.begin() + 1, det.end(), g);
 }
 
 namespace tpb {

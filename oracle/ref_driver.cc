@@ -9,11 +9,19 @@
 //                 --rdm r --out PREFIX [--load NAME] [--save NAME] [--dump_matrix_form_wf FILE]
 //       sbd::tpb::diag (sbdiag.h:113-477) -> PREFIX.energy.txt, PREFIX.wf.bin (unless
 //       another matrix-form file is named), PREFIX.rdm1.bin (2*L^2), PREFIX.rdm2.bin
-//       (4*L^4), PREFIX.density.bin (2*L); --load/--save are diag's loadname/savename
-//       (restart.h:20-67,414-657)
-//   sbd_ref sigma --fcidump F --adet A --bdet B --vec V --out PREFIX
+//       (4*L^4), PREFIX.density.bin (2*L), PREFIX.carryover.bin (carryover_bitstrings,
+//       sbdiag.h:436-453: int64 count, int64 words per string, then the words);
+//       --load/--save are diag's loadname/savename (restart.h:20-67,414-657)
+//   sbd_ref sigma --fcidump F --adet A --bdet B --vec V --out PREFIX [--corr 1]
 //       helpers (helper.h:447-589) + diagonal (tpb/qcham.h:314-366) + one direct mult
-//       (tpb/mult.h:376-632) -> PREFIX.hii.bin, PREFIX.sigma.bin, PREFIX.tables.bin
+//       (tpb/mult.h:376-632) -> PREFIX.hii.bin, PREFIX.sigma.bin, PREFIX.tables.bin,
+//       PREFIX.sigma_time.txt; with --corr 1 also Correlation (tpb/correlation.h:194-444)
+//       of the vector scaled to unit norm -> PREFIX.rdm1.bin, PREFIX.rdm2.bin,
+//       PREFIX.corr_time.txt
+//   sbd_ref shuffle --fcidump F --adet A --bdet A --out PREFIX
+//       ShuffleDet (determinants.h:589-594) of the alpha strings with the sample's seeds 1729
+//       and 137 (main.cc:123-128) -> PREFIX.shuffle.bin (int64 count, words, then the
+//       strings in the seed-1729 order, then in the seed-137 order)
 #include <cassert>
 #include <chrono>
 #include <cmath>
@@ -77,6 +85,14 @@ int main(int argc, char** argv) {
       write_bin(out + ".rdm2.bin", t);
     }
     write_bin(out + ".density.bin", density);
+    {
+      std::vector<int64_t> co;
+      co.push_back((int64_t)cobits.size());
+      co.push_back(cobits.empty() ? 0 : (int64_t)cobits[0].size());
+      for (auto& w : cobits)
+        for (size_t x : w) co.push_back((int64_t)x);
+      write_bin(out + ".carryover.bin", co);
+    }
     std::cout << " ref energy " << energy << std::endl;
   } else if (mode == "sigma") {
     double I0 = 0.0;
@@ -120,6 +136,36 @@ int main(int argc, char** argv) {
     std::ofstream e(out + ".sigma_time.txt");
     e.precision(17);
     e << std::chrono::duration<double>(t1 - t0).count() << "\n";
+    if (arg(argc, argv, "--corr", "0") == "1") {
+      double nn = 0.0;
+      for (double x : vec) nn += x * x;
+      for (double& x : vec) x /= std::sqrt(nn);
+      std::vector<std::vector<double>> one, two;
+      auto c0 = std::chrono::high_resolution_clock::now();
+      sbd::Correlation(vec, adet, bdet, sbd_data.bit_length, (size_t)L, 1, 1, helper, h_comm, b_comm, t_comm,
+                       one, two);
+      auto c1 = std::chrono::high_resolution_clock::now();
+      std::vector<double> o, t;
+      for (auto& v : one) o.insert(o.end(), v.begin(), v.end());
+      for (auto& v : two) t.insert(t.end(), v.begin(), v.end());
+      write_bin(out + ".rdm1.bin", o);
+      write_bin(out + ".rdm2.bin", t);
+      std::ofstream ec(out + ".corr_time.txt");
+      ec.precision(17);
+      ec << std::chrono::duration<double>(c1 - c0).count() << "\n";
+    }
+  }
+  if (mode == "shuffle") {
+    std::vector<int64_t> outw;
+    outw.push_back((int64_t)adet.size());
+    outw.push_back(adet.empty() ? 0 : (int64_t)adet[0].size());
+    for (unsigned int seed : {1729u, 137u}) {
+      auto d = adet;
+      sbd::ShuffleDet(d, seed);
+      for (auto& w : d)
+        for (size_t x : w) outw.push_back((int64_t)x);
+    }
+    write_bin(out + ".shuffle.bin", outw);
   }
   MPI_Finalize();
   return 0;

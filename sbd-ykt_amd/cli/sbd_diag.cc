@@ -1,8 +1,10 @@
 // sbd_diag -- command-line driver with the reference sample's interface
 // (samples/selected_basis_diagonalization/main.cc:17-226): the SBD options of
 // generate_sbd_data plus --fcidump, --adetfile (beta strings = alpha strings, as the
-// sample does), --bdetfile (separate beta strings), --carryoverfile, --rdmfiles 1,
-// --loadname / --savename (wavefunction restart files).
+// sample does), --bdetfile (separate beta strings), --carryoverfile,
+// --loadname / --savename (wavefunction restart files), --shuffle 1 (the strings after the
+// first in a seeded random order, main.cc:123-128).  With --rdm 1 it writes 1pRDM.txt and
+// 2pRDM.txt (spin-summed, main.cc:172-205) in the working directory.
 // Prints the energy, the density, the carry-over strings and, with --rdm 1, the
 // one-body + two-body energy identity.
 #include <cstdio>
@@ -33,6 +35,10 @@ int main(int argc, char** argv) {
     sbd::LoadAlphaDets(afile, adet, o.bit_length, L);
     if (bfile.empty()) bdet = adet;
     else sbd::LoadAlphaDets(bfile, bdet, o.bit_length, L);
+    if (o.do_shuffle != 0) {  // main.cc:123-128
+      sbd::ShuffleDet(adet, 1729);
+      sbd::ShuffleDet(bdet, 137);
+    }
     double energy = 0.0;
     std::vector<double> density;
     std::vector<std::vector<size_t>> cobits;
@@ -79,6 +85,23 @@ int main(int argc, char** argv) {
               e2 += 0.5 * eri[((size_t)(i * L + k) * L + j) * L + l] *
                     (two[0][x] + two[1][x] + two[2][x] + two[3][x]);
             }
+      // the spin-summed RDM dumps of main.cc:172-205: "i j value" and "i j k l value"
+      {
+        std::ofstream f1("1pRDM.txt");
+        f1.precision(16);
+        for (int i = 0; i < L; ++i)
+          for (int j = 0; j < L; ++j) f1 << i << " " << j << " " << one[0][i + L * j] + one[1][i + L * j] << "\n";
+        std::ofstream f2("2pRDM.txt");
+        f2.precision(16);
+        for (int i = 0; i < L; ++i)
+          for (int j = 0; j < L; ++j)
+            for (int k = 0; k < L; ++k)
+              for (int l = 0; l < L; ++l) {
+                const size_t x = i + (size_t)L * j + L2 * (k + (size_t)L * l);
+                f2 << i << " " << j << " " << k << " " << l << " " << two[0][x] + two[1][x] + two[2][x] + two[3][x]
+                   << "\n";
+              }
+      }
       std::cout << " One-Body energy = " << e1 << std::endl;
       std::cout << " Two-Body energy = " << e2 << std::endl;
       std::cout << " One-Body + Two-Body energy = " << e1 + e2 << " (+ core " << I0 << ")" << std::endl;

@@ -109,6 +109,19 @@ def lib():
     return _lib
 
 
+def device_memory():
+    """(free, total) device bytes of the current device, from the HIP runtime the library
+    loaded (torch ships another runtime of the same soname: it must not be initialised
+    after this library's)."""
+    lib()
+    hip = C.CDLL("libamdhip64.so.7")
+    free, total = C.c_size_t(), C.c_size_t()
+    rc = hip.hipMemGetInfo(C.byref(free), C.byref(total))
+    if rc != 0:
+        raise SbdgError(ERR_HIP, f"hipMemGetInfo failed ({rc})")
+    return free.value, total.value
+
+
 def _d(a):
     return a.ctypes.data_as(_D)
 

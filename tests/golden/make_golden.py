@@ -16,12 +16,23 @@ seeded recipe plus a checksum of them) next to the reference's outputs:
 * ``ref_n2_1em3.npz`` / ``ref_h2o_1em3.npz`` -- the reference datasets (data/n2, data/h2o)
   with the threshold-1e-3 string files: energy and 1-RDM from full-precision integrals,
   and the energy from integrals rounded to six digits (the published-table path).
+* ``ref_c3_rdm.npz`` -- the norb=16, 5+5 full-space workload (seed 3, 19.1M determinants):
+  one sigma and the 1-/2-RDMs (Correlation, tpb/correlation.h:194-444) of a seeded vector
+  (numpy default_rng(33).normal, scaled to unit norm), with checksums of the sigma.
+* ``ref_carryover.npz`` -- carry-over strings of diag (CarryOverAlphaDet, tpb/rdmat.h:60-132)
+  by threshold and by ratio on two small cases.
+* ``ref_shuffle.npz`` -- the N2 threshold-1e-3 alpha strings in the orders ShuffleDet
+  (determinants.h:589-594) gives them with the sample's seeds 1729 and 137 (main.cc:123-128).
+* ``ref_cpu_timings.json`` -- the reference's CPU path timed here (cores stated): the full
+  Davidson of the norb=12 workload, one sigma of the norb=26 4,000 x 4,000 workload, and the
+  sigma and Correlation of the norb=16 workload (from the ref_c3_rdm run).
 
 Usage: python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
 
 import gzip
+import json
 import os
 import shutil
 import subprocess
@@ -44,6 +55,44 @@ def run_ref(args, cwd):
     env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"),
                MKL_THREADING_LAYER="SEQUENTIAL")
     subprocess.check_call([REF] + args, cwd=cwd, env=env, stdout=subprocess.DEVNULL)
+
+
+def ref_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "8"))
+
+
+def write_inputs(tmp, wl):
+    L = wl.norb
+    fio.write_fcidump(os.path.join(tmp, "fcidump.txt"), L, wl.nalpha + wl.nbeta, wl.h1, wl.eri,
+                      core=wl.core, ms2=wl.nalpha - wl.nbeta)
+    fio.write_strings(os.path.join(tmp, "a.txt"), wl.astr, L)
+    fio.write_strings(os.path.join(tmp, "b.txt"), wl.bstr, L)
+    return ["--fcidump", "fcidump.txt", "--adet", "a.txt", "--bdet", "b.txt"]
+
+
+def update_timings(**entries):
+    path = os.path.join(HERE, "ref_cpu_timings.json")
+    t = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            t = json.load(f)
+    t.update(entries)
+    t["host"] = ("this container: %d-core CPU (%s), 1 MPI rank x OMP_NUM_THREADS threads, "
+                 "MKL sequential" % (os.cpu_count(), _cpu_name()))
+    with open(path, "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+        f.write("\n")
+
+
+def _cpu_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def rd(path, dtype=np.float64):
@@ -231,6 +280,117 @@ def case_dataset(tag, fcidump, adetfile, rounded_too=True):
         shutil.rmtree(tmp)
 
 
+def case_c3_rdm():
+    """One sigma and Correlation of the reference on a seeded unit vector over the norb=16,
+    5+5 full space (19.1M determinants); about two hours on 8 cores."""
+    wl = syn.config_workload("c3_L16_5e5e_full")
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        common = write_inputs(tmp, wl)
+        vec = np.random.default_rng(33).normal(size=wl.ndet)
+        vec.tofile(os.path.join(tmp, "vec.bin"))
+        run_ref(["sigma"] + common + ["--vec", "vec.bin", "--out", "s", "--corr", "1"], tmp)
+        sig = rd(os.path.join(tmp, "s.sigma.bin"))
+        L = wl.norb
+        with open(os.path.join(tmp, "s.sigma_time.txt")) as f:
+            t_sigma = float(f.readline())
+        with open(os.path.join(tmp, "s.corr_time.txt")) as f:
+            t_corr = float(f.readline())
+        res = dict(vec_seed=33, rdm1=rd(os.path.join(tmp, "s.rdm1.bin")).reshape(2, L * L),
+                   rdm2=rd(os.path.join(tmp, "s.rdm2.bin")).reshape(4, L ** 4),
+                   input_checksum=checksum(wl.h1, wl.eri, wl.astr.astype(np.float64), wl.bstr.astype(np.float64)),
+                   sigma_checksum=checksum(sig), sigma_norm=float(np.linalg.norm(sig)),
+                   sigma_dot_vec=float(sig @ vec), sigma_head=sig[:4096].copy())
+        np.savez_compressed(os.path.join(HERE, "ref_c3_rdm.npz"), **res)
+        update_timings(c3_L16_5e5e_full=dict(what="one sigma (mult) and one Correlation of a unit vector",
+                                             determinants=int(wl.ndet), sigma_s=t_sigma, correlation_s=t_corr,
+                                             threads=ref_threads()))
+        print("c3", wl.ndet, t_sigma, t_corr)
+    finally:
+        shutil.rmtree(tmp)
+
+
+def case_timing_c0():
+    """The reference's full Davidson on the norb=12, 4+4 full space at tol 1e-8 (timed)."""
+    wl = syn.config_workload("c0_L12_4e4e_full")
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        common = write_inputs(tmp, wl)
+        run_ref(["diag"] + common + ["--block", "10", "--iteration", "200", "--tolerance", "1e-8",
+                                     "--rdm", "0", "--out", "d"], tmp)
+        with open(os.path.join(tmp, "d.energy.txt")) as f:
+            e, t = float(f.readline()), float(f.readline())
+        update_timings(c0_L12_4e4e_full=dict(what="diag: tables, Davidson to tol 1e-8 (block 10), final energy, "
+                                                  "density", determinants=int(wl.ndet), diag_s=t, energy=e,
+                                             threads=ref_threads()))
+        print("c0", t, e)
+    finally:
+        shutil.rmtree(tmp)
+
+
+def case_timing_c1():
+    """One sigma of the reference on the norb=26, 5+5, 4,000 x 4,000 workload (timed)."""
+    wl = syn.config_workload("c1_L26_5e5e_4000")
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        common = write_inputs(tmp, wl)
+        np.random.default_rng(11).normal(size=wl.ndet).tofile(os.path.join(tmp, "vec.bin"))
+        run_ref(["sigma"] + common + ["--vec", "vec.bin", "--out", "s"], tmp)
+        with open(os.path.join(tmp, "s.sigma_time.txt")) as f:
+            t = float(f.readline())
+        update_timings(c1_L26_5e5e_4000=dict(what="one sigma (mult)", determinants=int(wl.ndet), sigma_s=t,
+                                             threads=ref_threads()))
+        print("c1", t)
+    finally:
+        shutil.rmtree(tmp)
+
+
+def case_carryover():
+    """Carry-over strings of diag by threshold (default 0.01 and 0.2) and by ratio (0.3) on
+    the asym_L14 and sym_L10 cases."""
+    cases = {"asym_L14": syn.Workload(*_sampled(14, 4, 3, 150, 120, 12)),
+             "sym_L10": syn.make_workload(10, 3, 3, 13)}
+    res = {}
+    for name, wl in cases.items():
+        tmp = tempfile.mkdtemp(prefix="golden_")
+        try:
+            common = write_inputs(tmp, wl)
+            for tag, flag in (("thr001", ["--carryover_threshold", "0.01"]),
+                              ("thr02", ["--carryover_threshold", "0.2"]),
+                              ("ratio03", ["--carryover_ratio", "0.3"])):
+                run_ref(["diag"] + common + ["--block", "10", "--iteration", "200", "--tolerance", "1e-10",
+                                             "--rdm", "0", "--out", "d"] + flag, tmp)
+                co = rd(os.path.join(tmp, "d.carryover.bin"), np.int64)
+                n, words = int(co[0]), int(co[1])
+                w = co[2:].reshape(n, words) if n else np.zeros((0, max(words, 1)), np.int64)
+                strs = np.array([sum(int(w[i, k]) << (20 * k) for k in range(words)) for i in range(n)],
+                                dtype=np.uint64)
+                res[f"{name}_{tag}"] = strs
+        finally:
+            shutil.rmtree(tmp)
+        print("carryover", name, {k: len(v) for k, v in res.items() if k.startswith(name)})
+    np.savez_compressed(os.path.join(HERE, "ref_carryover.npz"), **res)
+
+
+def case_shuffle():
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        shutil.copy(os.path.join(HERE, "n2_1em3-alpha.txt"), os.path.join(tmp, "a.txt"))
+        with gzip.open(os.path.join(HERE, "n2_fcidump.txt.gz"), "rb") as f, \
+                open(os.path.join(tmp, "fcidump.txt"), "wb") as g:
+            g.write(f.read())
+        run_ref(["shuffle", "--fcidump", "fcidump.txt", "--adet", "a.txt", "--bdet", "a.txt", "--out", "s"], tmp)
+        w = rd(os.path.join(tmp, "s.shuffle.bin"), np.int64)
+        n, words = int(w[0]), int(w[1])
+        orders = w[2:].reshape(2, n, words)
+        strs = [np.array([sum(int(o[i, k]) << (20 * k) for k in range(words)) for i in range(n)], dtype=np.uint64)
+                for o in orders]
+        np.savez_compressed(os.path.join(HERE, "ref_shuffle.npz"), seed1729=strs[0], seed137=strs[1])
+        print("shuffle", n)
+    finally:
+        shutil.rmtree(tmp)
+
+
 def copy_datasets():
     for src, dst in ((f"{REFDATA}/n2/fcidump.txt", "n2_fcidump.txt.gz"),
                      (f"{REFDATA}/h2o/fcidump.txt", "h2o_fcidump.txt.gz")):
@@ -252,3 +412,13 @@ if __name__ == "__main__":
         case_c0()
     if "restart" in want:
         case_restart()
+    if "carryover" in want:
+        case_carryover()
+    if "shuffle" in want:
+        case_shuffle()
+    if "timing_c0" in want:
+        case_timing_c0()
+    if "timing_c1" in want:
+        case_timing_c1()
+    if "c3_rdm" in want:
+        case_c3_rdm()

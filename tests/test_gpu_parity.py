@@ -537,6 +537,29 @@ def test_carryover_matches_row_weights():
     assert len(co2) == int(0.1 * s.na) and abs(tw - (1 - D[order[:len(co2)]].sum())) < 1e-12
 
 
+@pytest.mark.parametrize("name", ("asym_L14", "sym_L10"))
+def test_carryover_vs_reference(name):
+    """Carry-over strings against the ones the reference's diag wrote (CarryOverAlphaDet,
+    tpb/rdmat.h:60-132) by threshold (0.01, 0.2) and by ratio (0.3): the same strings in the
+    same order (heaviest rows first; the reference sorts with std::sort, so rows of equal
+    weight may come in either order)."""
+    g = load_golden("ref_carryover.npz")
+    _, case = small_case(name)
+    s = solver(case, max_it=200, max_nb=10, eps=1e-10)
+    s.run()
+    w = s.vector().reshape(s.na, s.nb)
+    weight = dict(zip(s.astr.tolist(), (w * w).sum(1)))
+    for tag, kw in (("thr001", dict(threshold=0.01, ratio=0.0)), ("thr02", dict(threshold=0.2, ratio=0.0)),
+                    ("ratio03", dict(ratio=0.3))):
+        want = g[f"{name}_{tag}"]
+        got, _ = s.carryover(**kw)
+        assert len(got) == len(want), tag
+        assert set(got.tolist()) == set(want.tolist()), tag
+        for x, y in zip(got.tolist(), want.tolist()):
+            assert x == y or abs(weight[x] - weight[y]) < 1e-12, (tag, x, y)
+    s.close()
+
+
 def test_errors_from_the_host():
     case = workload_small(6, 3, 2, 12, 9, 21)
     with pytest.raises(sbdg.SbdgError) as ex:

@@ -154,3 +154,38 @@ def test_c4_mixed_davidson_fits_hbm():
     assert fp64["davidson"] == N * (8 * 8 * 2 + 3 * 8)
     print(f"c4 plan: mixed {plan['total'] / 1e9:.1f} GB (build {plan['build'] / 1e9:.1f}, Davidson "
           f"{plan['davidson'] / 1e9:.1f}, RDM {plan['rdm'] / 1e9:.1f}); FP64 {fp64['total'] / 1e9:.1f} GB")
+
+
+SHUFFLE_PROG = r"""
+#include <cstdio>
+#include "sbd_tpb.h"
+int main(int argc, char** argv) {
+  std::vector<std::vector<size_t>> det;
+  sbd::LoadAlphaDets(argv[1], det, 20, 18);
+  for (unsigned seed : {1729u, 137u}) {
+    auto d = det;
+    sbd::ShuffleDet(d, seed);
+    for (auto& w : d) {
+      unsigned long long x = 0;
+      for (size_t k = 0; k < w.size(); ++k) x |= (unsigned long long)w[k] << (20 * k);
+      std::printf("%llu\n", x);
+    }
+  }
+  return 0;
+}
+"""
+
+
+def test_shuffle_matches_reference(tmp_path):
+    """--shuffle: sbd::ShuffleDet of the drop-in header puts the N2 threshold-1e-3 strings in
+    the orders the reference's ShuffleDet (determinants.h:589-594) gave them with the
+    sample's seeds 1729 and 137 (golden ref_shuffle.npz)."""
+    src = tmp_path / "shuffle.cc"
+    src.write_text(SHUFFLE_PROG)
+    exe = tmp_path / "shuffle"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = subprocess.check_output([str(exe), os.path.join(ROOT, "tests", "golden", "n2_1em3-alpha.txt")], text=True)
+    got = np.array([int(x) for x in out.split()], dtype=np.uint64).reshape(2, -1)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "ref_shuffle.npz"))
+    assert np.array_equal(got[0], g["seed1729"]) and np.array_equal(got[1], g["seed137"])
+    assert got[0][0] == got[0].min()  # the Hartree-Fock string stays first

@@ -221,3 +221,41 @@ def test_cli_savename_and_loadname(tmp_path):
                          timeout=120).stdout
     line = [x for x in out.splitlines() if "Energy =" in x][0]
     assert abs(float(line.split("=")[1]) - float(r["energy"])) < 1e-8
+
+
+@pytest.mark.gpu
+def test_cli_shuffle_and_rdm_dumps(tmp_path):
+    """The sample driver with --shuffle 1 --rdm 1 on the N2 threshold-1e-3 strings (the
+    strings after the first in ShuffleDet's order, main.cc:123-128): the energy is the
+    reference's full-precision one, and 1pRDM.txt / 2pRDM.txt (main.cc:172-205) hold the
+    spin-summed RDMs that give that energy through the energy identity."""
+    import gzip
+    import shutil
+
+    g = load_golden("ref_n2_1em3.npz")
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "n2_fcidump.txt.gz"), "rb") as f, \
+            open(tmp_path / "fcidump.txt", "wb") as o:
+        o.write(f.read())
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "n2_1em3-alpha.txt"), tmp_path / "a.txt")
+    exe = os.path.join(ROOT, "sbd-ykt_amd", "sbdg", "sbd_diag")
+    out = subprocess.run([exe, "--fcidump", "fcidump.txt", "--adetfile", "a.txt", "--block", "10", "--iteration",
+                          "100", "--tolerance", "1e-10", "--rdm", "1", "--shuffle", "1"], cwd=tmp_path, check=True,
+                         capture_output=True, text=True, timeout=300).stdout
+    e = float([x for x in out.splitlines() if "Energy =" in x][0].split("=")[1])
+    assert abs(e - float(g["energy"])) < 1e-8
+    L = 18
+    one = np.loadtxt(tmp_path / "1pRDM.txt")
+    two = np.loadtxt(tmp_path / "2pRDM.txt")
+    assert one.shape == (L * L, 3) and two.shape == (L ** 4, 5)
+    assert np.array_equal(one[:, 0], np.repeat(np.arange(L), L)) and np.array_equal(one[:, 1], np.tile(np.arange(L), L))
+    g1 = np.zeros((L, L))
+    g1[one[:, 0].astype(int), one[:, 1].astype(int)] = one[:, 2]       # g1[i, j] = gamma[i + L j]
+    assert np.abs(g1 - (g["rdm1"][0] + g["rdm1"][1]).reshape(L, L, order="F")).max() < 1e-7
+    i, j, k, l = (two[:, c].astype(int) for c in range(4))
+    G = np.zeros((L, L, L, L))
+    G[i, j, k, l] = two[:, 4]                                          # G[i,j,k,l] = Gamma[i+Lj+L^2k+L^3l]
+    assert abs(np.einsum("ijij->", G) - float(g["rdm2_sum_trace"])) < 1e-6
+    idline = [x for x in out.splitlines() if "One-Body + Two-Body" in x][0]
+    e12 = float(idline.split("=")[1].split("(")[0])
+    core = float(idline.split("core")[1].strip(" )"))
+    assert abs(core + e12 - e) < 1e-8
