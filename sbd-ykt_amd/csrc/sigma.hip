@@ -327,20 +327,6 @@ struct AbRows {
   }
 };
 
-// the (ai|..) slice of one row into LDS: element x of row g lives at V[G*x + g] and, with a
-// second copy (sb2 > 0), at V[G*(sb2 + x) + g]; ai < 0: a zero row
-template <class T>
-__device__ inline void ab_slice(int ai, int64_t LL, int64_t sb2, const T* __restrict__ eri, T* __restrict__ V,
-                                int g) {
-  constexpr int G = AbRows<T>::G;
-  const T* src = eri + (int64_t)(ai < 0 ? 0 : ai) * LL;
-  for (int x = threadIdx.x; x < LL; x += blockDim.x) {
-    const T v = ai < 0 ? T(0) : src[x];
-    V[G * x + g] = v;
-    if (sb2) V[G * (sb2 + x) + g] = v;
-  }
-}
-
 // The LDS vector at byte address a of the workgroup's LDS.  The alpha-beta kernel has no
 // static LDS, so its dynamic LDS starts at address 0 (checked at launch) and entries carry
 // absolute byte addresses.
@@ -403,6 +389,9 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
   using R = AbRows<T>;
   using RV = typename R::V;
   constexpr int G = R::G;
+  // positions per thread per staging round: two 16-byte vectors each (slice and ket), 8*SR
+  // VGPRs beside the accumulators (eight rounds' worth only where they fit without spills)
+  constexpr int SR = (sizeof(T) == 8 && M <= 20) ? 8 : 4;
   extern __shared__ __align__(16) unsigned char smem[];
   RV* Vv = reinterpret_cast<RV*>(smem);  // L*L slice G-vectors (twice when sb2 > 0)
   RV* cbv = Vv + kbase;                   // chunk + 1 G-vectors (the last one stays zero: padding)
@@ -444,6 +433,7 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
       // rows t..t+G-1; rows at or past q1 are absent (zero)
       T coef[G];
       const T* crow[G];
+      const T* srow[G];
       bool has[G];
       __syncthreads();  // previous rows done with LDS (and the previous chunk's sums are stored)
 #pragma unroll
@@ -452,31 +442,42 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
         has[g] = q < q1;
         coef[g] = T(0);
         crow[g] = c + ia * nb + j0;
+        srow[g] = eri;
         if (has[g]) {
           const uint32_t e = excSa[q];
           coef[g] = ((e >> 24) & 1u) ? T(-1) : T(1);
           crow[g] = c + (int64_t)colSa[q] * nb + j0;
-          ab_slice<T>((int)((e >> 6) & 63u) * L + (int)(e & 63u), LL, sb2, eri, Vs, g);  // a*L + i
-        } else {
-          ab_slice<T>(-1, LL, sb2, eri, Vs, g);
+          srow[g] = eri + (int64_t)((int)((e >> 6) & 63u) * L + (int)(e & 63u)) * LL;  // (ai|..), a*L + i
         }
       }
-      // the thread index, opaque here so that the fill's addresses are formed per step
-      // rather than held in registers through the products
+      // the G slices and the G ket rows' chunk into LDS: in rounds of SR positions per thread,
+      // every global load of a round issued before its LDS stores (the loads do not depend on
+      // each other), so a round costs one memory latency.  The thread index is opaque here so
+      // that the addresses are formed per step rather than held in registers through the
+      // products.
       int xt = (int)threadIdx.x;
       asm volatile("" : "+v"(xt));
-      for (int x0 = xt; x0 < nj; x0 += 4 * NT) {
-        RV v[4];
+      const int span = (int)max((int64_t)nj, LL);
+      for (int x0 = xt; x0 < span; x0 += SR * NT) {
+        RV sv[SR], kv[SR];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < SR; ++u) {
           const int x = x0 + u * NT;
 #pragma unroll
-          for (int g = 0; g < G; ++g)
-            v[u].v[g] = (has[g] && x < nj) ? __builtin_nontemporal_load(crow[g] + x) : T(0);
+          for (int g = 0; g < G; ++g) {
+            sv[u].v[g] = (has[g] && x < LL) ? srow[g][x] : T(0);
+            kv[u].v[g] = (has[g] && x < nj) ? __builtin_nontemporal_load(crow[g] + x) : T(0);
+          }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (x0 + u * NT < nj) cbv[x0 + u * NT] = v[u];
+        for (int u = 0; u < SR; ++u) {
+          const int x = x0 + u * NT;
+          if (x < LL) {
+            Vv[x] = sv[u];
+            if (sb2) Vv[sb2 + x] = sv[u];
+          }
+          if (x < nj) cbv[x] = kv[u];
+        }
       }
       __syncthreads();
       // the stream's batch counts, opaque to the compiler here: they are then reloaded for
