@@ -11,7 +11,8 @@ Also reported:
   * FP32 sigma throughput of the mixed path;
   * time-to-|dE| < 1e-6 Ha for the FP64 Davidson and the mixed Davidson at the workload's
     Davidson settings (SETTINGS), against the converged FP64 energy, and the mixed energy
-    error (--tts-starts 3: medians over three start vectors); for the largest workload the
+    error: medians over three start vectors (the first determinant, two seeded random
+    vectors; --tts-starts 1 keeps the first); for the largest workload the
     FP64 reference run takes the largest subspace that fits the free HBM, and the RDMs and
     their backward are timed after the mixed run;
   * roofline of the dominant sigma kernel: its algorithmic HBM bytes per launch (model
@@ -215,7 +216,8 @@ def main():
     ap.add_argument("--no-tts", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sequential", action="store_true", help="time sigma with its kernels in one stream")
-    ap.add_argument("--tts-starts", type=int, default=1, help="start vectors of the time-to-solution runs (1-3)")
+    ap.add_argument("--tts-starts", type=int, default=3,
+                    help="start vectors of the time-to-solution runs (1-3; speedups are medians over them)")
     ap.add_argument("--davidson-steps", action="store_true",
                     help="only run 20 FP64 and 24 mixed Davidson steps (for a kernel trace of the vector passes)")
     ap.add_argument("--copy-calibration", action="store_true",

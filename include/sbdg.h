@@ -53,7 +53,8 @@ typedef struct {
   int precision;      /* 0: FP64 sigma; 1: mixed (FP32 sigma + FP64 refresh) */
   int refresh_k;      /* mixed: FP64 sigma of the Ritz vector every refresh_k steps (4);
                          <= 0: when the FP32 residual has fallen tenfold since the last
-                         exact one or is below eps (a full basis then restarts without) */
+                         exact one or is below eps, or the FP32 Ritz value moved less than
+                         1e-6 Ha in one step (a full basis then restarts without) */
   double target_de;   /* mixed: also stop when |E - E_prev refresh| < target_de (0: off) */
   int verbose;        /* 1: one line per Davidson step on stdout */
   uint64_t seed;      /* init==1: random start vector seed */

@@ -653,12 +653,15 @@ void davidson(sbdg_ctx* c) {
   double prev_exact = NAN;
   // mixed with refresh_k <= 0: refresh when the FP32 residual estimate has fallen by
   // ADAPT_DROP since the last exact residual (the first estimate before any), when it
-  // is below eps (to confirm convergence), or after ADAPT_MAX steps; a full basis
-  // restarts without a refresh
+  // is below eps (to confirm convergence), or when the FP32 Ritz value moved less than
+  // ADAPT_DE in one step (the energy has settled: an exact one is worth its FP64 sigma);
+  // a full basis restarts without a refresh.  FP32 Ritz values sit within ~1e-7 Ha of the
+  // exact Rayleigh quotients on the benchmark workloads, so ADAPT_DE = 1e-6 Ha triggers only
+  // once the energy has converged to about the accuracy asked of the mixed path.
   const bool adaptive = mixed && o.refresh_k <= 0;
   constexpr double ADAPT_DROP = 0.1;
-  constexpr int ADAPT_MAX = 32;
-  double r_ref = -1.0;
+  constexpr double ADAPT_DE = 1e-6;
+  double r_ref = -1.0, theta32_prev = NAN;
   while (!converged && !stop && steps < max_steps) {
     const double ts = now();
     // sigma of the newest basis vector and column h of the projected matrix
@@ -693,7 +696,9 @@ void davidson(sbdg_ctx* c) {
     else ritz(h);
     if (adaptive) {
       if (r_ref < 0.0) r_ref = rnorm;
-      do_refresh = rnorm < o.eps || (since >= 2 && rnorm < ADAPT_DROP * r_ref) || since >= ADAPT_MAX;
+      do_refresh = rnorm < o.eps || (since >= 2 && rnorm < ADAPT_DROP * r_ref) ||
+                   (since >= 2 && std::fabs(theta - theta32_prev) < ADAPT_DE);
+      theta32_prev = do_refresh ? NAN : theta;
       if (do_refresh && !w_valid) ritz(h);  // the fused step formed no Ritz vector
     }
     if (!mixed) {
