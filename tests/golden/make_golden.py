@@ -17,8 +17,8 @@ seeded recipe plus a checksum of them) next to the reference's outputs:
   with the threshold-1e-3 string files: energy and 1-RDM from full-precision integrals,
   and the energy from integrals rounded to six digits (the published-table path).
 * ``ref_c3_rdm.npz`` -- the norb=16, 5+5 full-space workload (seed 3, 19.1M determinants):
-  one sigma and the 1-/2-RDMs (Correlation, tpb/correlation.h:194-444) of a seeded vector
-  (numpy default_rng(33).normal, scaled to unit norm), with checksums of the sigma.
+  one sigma of a seeded vector (numpy default_rng(33).normal), with checksums, and the
+  1-/2-RDMs (Correlation, tpb/correlation.h:194-444) of that vector scaled to unit norm.
 * ``ref_carryover.npz`` -- carry-over strings of diag (CarryOverAlphaDet, tpb/rdmat.h:60-132)
   by threshold and by ratio on two small cases.
 * ``ref_shuffle.npz`` -- the N2 threshold-1e-3 alpha strings in the orders ShuffleDet

@@ -6,7 +6,8 @@ need (through the C-ABI).
 * the full 40,000 x 40,000 workload: sigma rows against the oracle, symmetry, FP32 rows;
 * norb=26, 5+5, 4,000 x 4,000 at its stated settings (tol 1e-8, subspace 24, restart to 4):
   the converged residual recomputed on oracle rows, and the mixed energy;
-* norb=16, 5+5 full space (19.1M determinants): RDM invariants and bitwise reproducibility;
+* norb=16, 5+5 full space (19.1M determinants): RDM invariants and bitwise reproducibility,
+  and sigma and the RDMs of a seeded vector against the reference's own run;
 * the mixed energy against the reference's own energies (golden fixtures), with the
   refresh every 4 steps, scheduled by the residual, with the large workload's subspace
   (8, restart to 3) and stopped by target_de;
@@ -14,7 +15,8 @@ need (through the C-ABI).
 
 Bars: FP64 sigma within 1e-12 of the oracle (relative to its largest element), FP32 within
 2e-5; FP64 energies within 1e-8 Ha; mixed energies within 1e-6 Ha of the reference; RDMs
-within 1e-7; invariants within 1e-10; RDMs bit-identical between two evaluations.
+within 1e-7 (1e-10 when both sides use the same vector); invariants within 1e-10; RDMs
+bit-identical between two evaluations.
 """
 import time
 
@@ -171,6 +173,40 @@ def test_c3_rdm_invariants_and_reproducibility():
     assert abs(energy_from_rdm(wl.core, wl.h1, wl.eri, one, two) - e) < 1e-9
     one2, two2 = s.rdm(compute=True)
     assert np.array_equal(one, one2) and np.array_equal(two, two2)
+    s.close()
+
+
+def weighted_sum(a):
+    """The fixtures' checksum: sum of a[i] cos(0.618034 i)."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    return float(a @ np.cos(np.arange(a.size) * 0.618034))
+
+
+def test_c3_sigma_and_rdm_vs_reference():
+    """norb=16, 5+5 full space (19.1M determinants) against the reference's own run (mult,
+    tpb/mult.h, and Correlation, tpb/correlation.h:194-444, on one rank): sigma of the seeded
+    vector numpy default_rng(33).normal, and the 1-/2-RDMs of that vector scaled to unit norm.
+    The vector enters as the Davidson start with a tolerance so loose that Davidson stops at
+    it, as the reference's loadname path does."""
+    g = load_golden("ref_c3_rdm.npz")
+    wl = syn.config_workload("c3_L16_5e5e_full")
+    inputs = sum(weighted_sum(x) for x in (wl.h1, wl.eri, wl.astr.astype(np.float64), wl.bstr.astype(np.float64)))
+    assert abs(inputs - float(g["input_checksum"])) < 1e-9 * abs(float(g["input_checksum"]))
+    vec = np.random.default_rng(int(g["vec_seed"])).normal(size=wl.ndet)
+    s = sbdg.Solver(wl.norb, wl.core, wl.h1, wl.eri, wl.astr, wl.bstr, max_it=1, max_nb=2, eps=1e10, init=2,
+                    do_rdm=1)
+    s.set_vector(vec)
+    e = s.run()
+    nn = float(vec @ vec)
+    sig = s.sigma(vec)
+    scale = np.abs(sig).max()
+    assert np.abs(sig[:4096] - g["sigma_head"]).max() < 1e-12 * scale
+    assert abs(float(sig @ vec) - float(g["sigma_dot_vec"])) < 1e-10 * abs(float(g["sigma_dot_vec"]))
+    assert abs(float(np.linalg.norm(sig)) - float(g["sigma_norm"])) < 1e-10 * float(g["sigma_norm"])
+    assert abs(weighted_sum(sig) - float(g["sigma_checksum"])) < 1e-9 * float(g["sigma_norm"])
+    assert abs(e - float(g["sigma_dot_vec"]) / nn) < 1e-9  # the energy of the start vector
+    one, two = s.rdm()
+    rdm_close(one, two, g["rdm1"], g["rdm2"], 1e-10, mixed_sum=True)
     s.close()
 
 
