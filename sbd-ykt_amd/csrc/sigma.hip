@@ -395,7 +395,6 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
   extern __shared__ __align__(16) unsigned char smem[];
   RV* Vv = reinterpret_cast<RV*>(smem);  // L*L slice G-vectors (twice when sb2 > 0)
   RV* cbv = Vv + kbase;                   // chunk + 1 G-vectors (the last one stays zero: padding)
-  T* Vs = reinterpret_cast<T*>(Vv);
   const int64_t LL = (int64_t)L * L;
   const int NT = blockDim.x;
   const int64_t npos = groups * 64;
