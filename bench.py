@@ -12,9 +12,11 @@ Also reported:
   * time-to-|dE| < 1e-6 Ha for the FP64 Davidson and the mixed Davidson at the workload's
     Davidson settings (SETTINGS), against the converged FP64 energy, and the mixed energy
     error: medians over three start vectors (the first determinant, two seeded random
-    vectors; --tts-starts 1 keeps the first); for the largest workload the
-    FP64 reference run takes the largest subspace that fits the free HBM, and the RDMs and
-    their backward are timed after the mixed run;
+    vectors; --tts-starts 1 keeps the first); the mixed runs refresh the exact sigma every
+    refresh_k steps and, second, on the residual's schedule (refresh_k = 0; also every 8
+    steps from the first determinant, except for the largest workload); for the largest
+    workload the FP64 reference run takes the largest subspace that fits the free HBM, and
+    the RDMs and their backward are timed after the mixed runs;
   * roofline of the dominant sigma kernel: its algorithmic HBM bytes per launch (model
     below) over its average duration from HIP events on the library stream, taken from
     sigma builds with the kernels in sequence (the timed builds run the alpha-beta kernel
@@ -331,7 +333,7 @@ def main():
                 # tables; the mixed run at its own settings
                 nb64 = fp64_subspace(s, st, na, nb, L, cfg["max_nb"])
                 r = dict(start=name, fp64_max_nb=nb64)
-                ks = (cfg["refresh_k"],)
+                ks = (cfg["refresh_k"], 0)
             else:
                 nb64 = cfg["max_nb"]
                 r = dict(start=name)
