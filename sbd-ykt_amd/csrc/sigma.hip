@@ -367,7 +367,7 @@ __device__ inline void ab_batch(const uint32_t (&p)[4], T (&s)[AbRows<T>::G]) {
 // G-vectors, each lane walks its ELL entries for that chunk (the lane -> column map is per
 // chunk, so lanes of a group carry lists of similar length), and the chunk's sums are
 // added into sigma before the next chunk (one owner per element: a column belongs to one
-// tile).  A wave's entries of a chunk are one stream (its groups one after another, four
+// tile; with one tile per row, through LDS in column order).  A wave's entries of a chunk are one stream (its groups one after another, four
 // rows per batch); the next batch is loaded while the current one is used, across group
 // boundaries, and after the last batch the stream's first batch is loaded again for the
 // next ket rows.
@@ -510,20 +510,54 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
     // opaque again: the column map is read here, not before the products
     int ft = (int)threadIdx.x;
     asm volatile("" : "+v"(ft));
+    const bool first = assign && k == 0;  // every column has a position in chunk 0
+    if (ntiles == 1) {
+      // One tile holds the whole row, each column at exactly one position of this chunk: the
+      // sums go through LDS (the ket vectors' region, idle now; the zero slot after it stays
+      // untouched) one block of columns at a time, then into sigma with coalesced
+      // read-modify-writes.  Scattered 8-byte ones fetch and write back whole cache lines
+      // once the rows in flight outgrow the L2 (at 40,000 columns they moved 4.7x the
+      // kernel's algorithmic bytes).  Each element is read and written by the same thread in
+      // every chunk.
+      T* rowbuf = reinterpret_cast<T*>(cbv);
+      const int64_t cap = chunk * G;
+      T* out = sigma + ia * nb;
+      for (int64_t b0 = 0; b0 < nb; b0 += cap) {
+        const int64_t b1 = min(nb, b0 + cap);
+        __syncthreads();  // the products, or the previous block's readers, are done with LDS
+        int fb = ft;  // opaque per block: the column map is read per block, not held in M registers
+        asm volatile("" : "+v"(fb));
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      const int64_t pos = col0 + (int64_t)m * NT + ft;
-      if ((pos >> 6) < groups) {
-        const int32_t ib = colperm[(int64_t)k * npos + pos];
-        if (ib >= 0) {
-          if (assign && k == 0) sigma[ia * nb + ib] = acc[m];  // every column has a position in chunk 0
-          else sigma[ia * nb + ib] += acc[m];
+        for (int m = 0; m < M; ++m) {
+          const int64_t pos = (int64_t)m * NT + fb;
+          if ((pos >> 6) < groups) {
+            const int32_t ib = colperm[(int64_t)k * npos + pos];
+            if (ib >= b0 && ib < b1) rowbuf[ib - b0] = acc[m];
+          }
+        }
+        __syncthreads();
+        for (int64_t x = threadIdx.x; x < b1 - b0; x += NT) {
+          if (first) out[b0 + x] = rowbuf[x];
+          else out[b0 + x] += rowbuf[x];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int64_t pos = col0 + (int64_t)m * NT + ft;
+        if ((pos >> 6) < groups) {
+          const int32_t ib = colperm[(int64_t)k * npos + pos];
+          if (ib >= 0) {
+            if (first) sigma[ia * nb + ib] = acc[m];
+            else sigma[ia * nb + ib] += acc[m];
+          }
         }
       }
     }
-    // a column's owner thread changes from chunk to chunk: the next chunk's read-modify-write
-    // of an element must see this chunk's store.  The barriers inside the row loop order
-    // them only when the bra row has alpha singles; this one also covers rows without.
+    // with several tiles a column's owner thread changes from chunk to chunk: the next
+    // chunk's read-modify-write of an element must see this chunk's store.  The barriers
+    // inside the row loop order them only when the bra row has alpha singles; this one also
+    // covers rows without (and the LDS reads of the last flush block).
     if (nch > 1) __syncthreads();
   }
 }
