@@ -513,14 +513,15 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
     const bool first = assign && k == 0;  // every column has a position in chunk 0
     if (ntiles == 1) {
       // One tile holds the whole row, each column at exactly one position of this chunk: the
-      // sums go through LDS (the ket vectors' region, idle now; the zero slot after it stays
-      // untouched) one block of columns at a time, then into sigma with coalesced
+      // sums go through LDS (the slices and ket vectors, idle now and staged again for the
+      // next rows; the zero slot after them stays untouched) one block of columns at a time,
+      // then into sigma with coalesced
       // read-modify-writes.  Scattered 8-byte ones fetch and write back whole cache lines
       // once the rows in flight outgrow the L2 (at 40,000 columns they moved 4.7x the
       // kernel's algorithmic bytes).  Each element is read and written by the same thread in
       // every chunk.
-      T* rowbuf = reinterpret_cast<T*>(cbv);
-      const int64_t cap = chunk * G;
+      T* rowbuf = reinterpret_cast<T*>(Vv);
+      const int64_t cap = (kbase + chunk) * G;
       T* out = sigma + ia * nb;
       for (int64_t b0 = 0; b0 < nb; b0 += cap) {
         const int64_t b1 = min(nb, b0 + cap);
