@@ -252,8 +252,7 @@ void build(sbdg_ctx* c) {
   } else {
     c->Zbuf.free();
   }
-  build_ab(c->Sb, L, std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - (int64_t)L * L)),
-           c->ab64, c->st);
+  build_ab(c->Sb, L, rdm_chunk(nb, L), c->ab64, c->st);
   build_ell(c->Sb, L, choose_chunk(nb, L), 8, c->ell, c->st);
   {
     int nt64, m64, nt32, m32;
@@ -348,7 +347,7 @@ int64_t build_bytes(const sbdg_options& o, int L, int64_t na, int64_t nb, const 
   // alpha-beta tables: RDM pack and chunk pointers; ELL entries (a group pads to its longest
   // list, at most the group's whole sum, plus three rows), lane maps, stream offsets
   const int64_t Sb = nnz[2];
-  const int64_t ach = std::min<int64_t>(std::max<int64_t>(nb, 1), std::max<int64_t>(256, 8192 - L2));
+  const int64_t ach = rdm_chunk(nb, L);
   b += 4 * Sb + 4 * (nb * ((nb + ach - 1) / ach + 1) + 1);
   const int64_t nch = (nb + std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32) - 1) /
                       std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32);

@@ -94,6 +94,65 @@ __global__ void __launch_bounds__(RT) k_gram(int64_t ncols, int L, const int64_t
   }
 }
 
+// g[q] = <X[r,:], X[col[q],:]> for the pairs of row r, GQ pairs per pass: row r is read
+// once per pass for all of them and their sums are reduced together (one barrier pair per
+// pass instead of per pair).  Each sum's order is fixed (a thread's columns in order, then
+// a shuffle tree, then the waves in order), so g does not depend on timing.  VEC: rows of
+// an even length, read two columns at a time.
+constexpr int GQ = 8;
+template <bool VEC>
+__global__ void __launch_bounds__(RT) k_gram_pairs(int64_t ncols, const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const double* __restrict__ X,
+                                                   double* __restrict__ g) {
+  __shared__ double sh[RT / 64][GQ];
+  const int64_t r = blockIdx.x;
+  const double* xr = X + r * ncols;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t q1 = rowptr[r + 1];
+  for (int64_t q0 = rowptr[r]; q0 < q1; q0 += GQ) {
+    const int nq = (int)min<int64_t>(GQ, q1 - q0);
+    const double* xq[GQ];
+    double s[GQ];
+#pragma unroll
+    for (int u = 0; u < GQ; ++u) {
+      xq[u] = X + (int64_t)col[q0 + min(u, nq - 1)] * ncols;
+      s[u] = 0.0;
+    }
+    if (VEC) {
+      for (int64_t cc = 2 * (int64_t)threadIdx.x; cc < ncols; cc += 2 * RT) {
+        const double2 x = *reinterpret_cast<const double2*>(xr + cc);
+#pragma unroll
+        for (int u = 0; u < GQ; ++u)
+          if (u < nq) {
+            const double2 y = *reinterpret_cast<const double2*>(xq[u] + cc);
+            s[u] += x.x * y.x;
+            s[u] += x.y * y.y;
+          }
+      }
+    } else {
+      for (int64_t cc = threadIdx.x; cc < ncols; cc += RT) {
+        const double x = xr[cc];
+#pragma unroll
+        for (int u = 0; u < GQ; ++u)
+          if (u < nq) s[u] += x * xq[u][cc];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GQ; ++u)
+      for (int o = 32; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, 64);
+    __syncthreads();  // the previous pass's readers of sh are done
+    if (lane == 0)
+#pragma unroll
+      for (int u = 0; u < GQ; ++u) sh[wid][u] = s[u];
+    __syncthreads();
+    if ((int)threadIdx.x < nq) {
+      double t = 0.0;
+      for (int w = 0; w < RT / 64; ++w) t += sh[w][threadIdx.x];
+      g[q0 + threadIdx.x] = t;
+    }
+  }
+}
+
 // per row: rsq[r] = sum_c X^2, P[r*L+k] = sum_c X[r,c]^2 n_k(cstr[c])
 __global__ void __launch_bounds__(RT) k_rowocc(int64_t ncols, int L, const double* __restrict__ X,
                                                const uint64_t* __restrict__ cstr, double* __restrict__ rsq,
@@ -326,6 +385,61 @@ __global__ void __launch_bounds__(RT) k_ab_rdm(int64_t nb, int L, const int32_t*
   for (int x = threadIdx.x; x < LL; x += RT) part[(int64_t)w * LL + x] = fx_get(T[x]);
 }
 
+// The same for rows of at most RTA*R columns: 1024 threads, thread t owning columns
+// t + RTA*r, whose bra values W[ia, ib] (signed) stay in registers for the whole alpha pair,
+// so each ket chunk costs its own values and the chunk pointers only.  With the chunk sized
+// to the whole LDS this reads a pair's bra row once instead of once per chunk.
+constexpr int RTA = 1024;
+template <int R>
+__global__ void __launch_bounds__(RTA) k_ab_rdm_reg(int64_t nb, int L, const int32_t* __restrict__ wseg,
+                                                    const int32_t* __restrict__ perm,
+                                                    const int64_t* __restrict__ rowof,
+                                                    const int32_t* __restrict__ colSa,
+                                                    const uint32_t* __restrict__ excSa,
+                                                    const uint32_t* __restrict__ pack,
+                                                    const int32_t* __restrict__ cptr, int nch, int64_t chunk,
+                                                    const double* __restrict__ W, double* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* cb = reinterpret_cast<double*>(smem);
+  unsigned long long* T = reinterpret_cast<unsigned long long*>(cb + chunk);
+  const int LL = L * L;
+  const int w = blockIdx.x;
+  for (int x = threadIdx.x; x < LL; x += RTA) T[x] = 0ull;
+  for (int32_t e = wseg[w]; e < wseg[w + 1]; ++e) {
+    const int32_t q = perm[e];
+    const int64_t ia = rowof[q], ja = colSa[q];
+    const double sa = ((excSa[q] >> 24) & 1u) ? -1.0 : 1.0;
+    double wb[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t ib = (int64_t)r * RTA + threadIdx.x;
+      wb[r] = ib < nb ? sa * W[ia * nb + ib] : 0.0;
+    }
+    for (int k = 0; k < nch; ++k) {
+      const int64_t j0 = (int64_t)k * chunk, j1 = min(nb, j0 + chunk);
+      __syncthreads();  // the previous chunk's products are done with cb
+      for (int64_t x = threadIdx.x; x < j1 - j0; x += RTA) cb[x] = W[ja * nb + j0 + x];
+      __syncthreads();
+      // opaque per chunk: the chunk-pointer addresses are formed here, not held per column
+      int tt = (int)threadIdx.x;
+      asm volatile("" : "+v"(tt));
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (wb[r] == 0.0) continue;  // also the columns past nb
+        const int32_t* cp = cptr + ((int64_t)r * RTA + tt) * (nch + 1) + k;
+        const int e1 = cp[1];
+        for (int ee = cp[0]; ee < e1; ++ee) {
+          const uint32_t p = pack[ee];
+          const int jb = (int)(p & 0x3FFFFu) - (int)j0;
+          fx_add(&T[(p >> 18) & 0xFFFu], (((p >> 30) & 1u) ? -wb[r] : wb[r]) * cb[jb]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < LL; x += RTA) part[(int64_t)w * LL + x] = fx_get(T[x]);
+}
+
 // sum the chunk partials of each key (a,i) and write Gamma_ab[a,b,i,j], Gamma_ba[b,a,j,i]
 __global__ void k_ab_slots(int L, const int32_t* __restrict__ kfirst, const double* __restrict__ part,
                            double* __restrict__ two) {
@@ -419,8 +533,10 @@ void doubles_part(const Table& t, int L, int s, const double* X, int64_t ncols, 
   if (t.nnz == 0) return;
   DBuf<double> g;
   g.alloc(t.nnz);
-  hipLaunchKernelGGL(k_gram<false>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, L, t.rowptr.p, t.col.p, X, nullptr,
-                     g.p, nullptr);
+  if (ncols % 2 == 0)
+    hipLaunchKernelGGL(k_gram_pairs<true>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, t.rowptr.p, t.col.p, X, g.p);
+  else
+    hipLaunchKernelGGL(k_gram_pairs<false>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, t.rowptr.p, t.col.p, X, g.p);
   DBuf<uint32_t> keys, ks;
   DBuf<int32_t> idx, perm, flag, pos, seg;
   keys.alloc(t.nnz);
@@ -537,9 +653,25 @@ void rdm_compute(const RdmPlan& P, const double* W, double* WT, double* one, dou
     if (nw) SBDG_CHECK(hipMemcpyAsync(dwk.p, wkey.data(), sizeof(int32_t) * nw, hipMemcpyHostToDevice, st));
     SBDG_CHECK(hipMemcpyAsync(dkf.p, kfirst.data(), sizeof(int32_t) * (L2 + 1), hipMemcpyHostToDevice, st));
     const size_t lds = sizeof(double) * (size_t)(P.ab->chunk + L2);
-    if (nw)
-      hipLaunchKernelGGL(k_ab_rdm, dim3(nw), dim3(RT), lds, st, nb, L, dws.p, dwk.p, perm.p, rowof.p, sa.col.p,
-                         sa.exc.p, P.ab->pack.p, P.ab->cptr.p, P.ab->nch, P.ab->chunk, W, part.p);
+    if (nw) {
+      const int64_t R = (nb + RTA - 1) / RTA;  // bra values per thread
+#define SBDG_ABR(RR)                                                                                          \
+  hipLaunchKernelGGL(k_ab_rdm_reg<RR>, dim3(nw), dim3(RTA), lds, st, nb, L, dws.p, perm.p, rowof.p, sa.col.p, \
+                     sa.exc.p, P.ab->pack.p, P.ab->cptr.p, P.ab->nch, P.ab->chunk, W, part.p)
+      if (R <= 2) SBDG_ABR(2);
+      else if (R <= 4) SBDG_ABR(4);
+      else if (R <= 8) SBDG_ABR(8);
+      else if (R <= 12) SBDG_ABR(12);
+      else if (R <= 16) SBDG_ABR(16);
+      else if (R <= 20) SBDG_ABR(20);
+      else if (R <= 24) SBDG_ABR(24);
+      else if (R <= 32) SBDG_ABR(32);
+      else if (R <= 40) SBDG_ABR(40);
+      else
+        hipLaunchKernelGGL(k_ab_rdm, dim3(nw), dim3(RT), lds, st, nb, L, dws.p, dwk.p, perm.p, rowof.p, sa.col.p,
+                           sa.exc.p, P.ab->pack.p, P.ab->cptr.p, P.ab->nch, P.ab->chunk, W, part.p);
+#undef SBDG_ABR
+    }
     hipLaunchKernelGGL(k_ab_slots, dim3((unsigned)L2), dim3(256), 0, st, L, dkf.p, part.p, two);
     SBDG_CHECK(hipGetLastError());
     SBDG_CHECK(hipStreamSynchronize(st));

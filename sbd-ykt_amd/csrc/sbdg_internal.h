@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <string>
 #include <vector>
@@ -257,6 +258,12 @@ inline int64_t ab_second_slice(int L) {
   const int64_t LL = (int64_t)L * L;
   const int64_t b = LL + (24 - LL % 16) % 16;  // b % 16 == 8
   return b + LL <= 4096 ? b : 0;
+}
+// Ket chunk of the RDM's alpha-beta kernel: the chunk's FP64 values and the L*L
+// accumulators of one orbital pair fill a CU's 160 KiB of LDS.
+inline int64_t rdm_chunk(int64_t nb, int L) {
+  const int64_t ch = std::max<int64_t>(256, (int64_t)(160 * 1024 / 8) - (int64_t)L * L - 8);
+  return std::min<int64_t>(std::max<int64_t>(nb, 1), ch);
 }
 inline int64_t ab_slice_span(int L) {
   const int64_t b = ab_second_slice(L);
