@@ -55,42 +55,30 @@ __device__ inline double block_sum_d(double v, double* sh) {
   return t;
 }
 
-// g[q] = <X[r,:], X[col[q],:]>; OCC: gocc[q*L+k] = sum_c X[r,c] X[col[q],c] n_k(cstr[c])
-template <bool OCC>
-__global__ void __launch_bounds__(RT) k_gram(int64_t ncols, int L, const int64_t* __restrict__ rowptr,
-                                             const int32_t* __restrict__ col, const double* __restrict__ X,
-                                             const uint64_t* __restrict__ cstr, double* __restrict__ g,
-                                             double* __restrict__ gocc) {
-  __shared__ double sh[RT / 64];
-  __shared__ unsigned long long acc[64];
+// first entry of [q0, q1) (partners in ascending order) whose partner lies above r
+__device__ inline int64_t first_above(int64_t q0, int64_t q1, const int32_t* __restrict__ col, int64_t r) {
+  while (q0 < q1) {
+    const int64_t mid = (q0 + q1) >> 1;
+    if ((int64_t)col[mid] <= r) q0 = mid + 1;
+    else q1 = mid;
+  }
+  return q0;
+}
+
+// The row-pair sums are symmetric in the two rows (gocc too: its weights belong to the
+// columns), and every pair (r, c) of an excitation table has its mirror (c, r), so the
+// Gram kernels form only the pairs with c > r; this copies them to the pairs with c < r
+// (the same products in the same order: the same bits).
+__global__ void k_gram_mirror(int L, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                              double* __restrict__ g, double* __restrict__ gocc) {
   const int64_t r = blockIdx.x;
-  const double* xr = X + r * ncols;
-  for (int64_t q = rowptr[r]; q < rowptr[r + 1]; ++q) {
-    const double* xq = X + (int64_t)col[q] * ncols;
-    if (OCC) {
-      if (threadIdx.x < 64) acc[threadIdx.x] = 0ull;
-      __syncthreads();
-    }
-    double s = 0.0;
-    for (int64_t cc = threadIdx.x; cc < ncols; cc += RT) {
-      const double p = xr[cc] * xq[cc];
-      s += p;
-      if (OCC && p != 0.0) {
-        uint64_t y = cstr[cc];
-        while (y) {
-          const int k = __builtin_ctzll(y);
-          y &= y - 1;
-          fx_add(&acc[k], p);
-        }
-      }
-    }
-    s = block_sum_d(s, sh);
-    if (threadIdx.x == 0) g[q] = s;
-    if (OCC) {
-      __syncthreads();
-      if (threadIdx.x < L) gocc[q * L + threadIdx.x] = fx_get(acc[threadIdx.x]);
-      __syncthreads();
-    }
+  const int64_t q1 = first_above(rowptr[r], rowptr[r + 1], col, r);
+  for (int64_t q = rowptr[r] + threadIdx.x; q < q1; q += blockDim.x) {
+    const int64_t c = col[q];
+    const int64_t m = first_above(rowptr[c], rowptr[c + 1], col, r - 1);  // the entry of r in row c
+    g[q] = g[m];
+    if (gocc)
+      for (int k = 0; k < L; ++k) gocc[q * L + k] = gocc[m * L + k];
   }
 }
 
@@ -109,7 +97,7 @@ __global__ void __launch_bounds__(RT) k_gram_pairs(int64_t ncols, const int64_t*
   const double* xr = X + r * ncols;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t q1 = rowptr[r + 1];
-  for (int64_t q0 = rowptr[r]; q0 < q1; q0 += GQ) {
+  for (int64_t q0 = first_above(rowptr[r], q1, col, r); q0 < q1; q0 += GQ) {
     const int nq = (int)min<int64_t>(GQ, q1 - q0);
     const double* xq[GQ];
     double s[GQ];
@@ -149,6 +137,90 @@ __global__ void __launch_bounds__(RT) k_gram_pairs(int64_t ncols, const int64_t*
       double t = 0.0;
       for (int w = 0; w < RT / 64; ++w) t += sh[w][threadIdx.x];
       g[q0 + threadIdx.x] = t;
+    }
+  }
+}
+
+// g[q] as k_gram_pairs and gocc[q*L + k] = sum_c X[r,c] X[col[q],c] n_k(cstr[c]) for the
+// pairs of row r, GQO pairs per pass.  The occupations enter relative to one string of the
+// column set, ref = cstr[0]: n_k(c) = n_k(ref) + d_k(c), d_k = +-1 only where c's
+// occupation differs from ref's (twice c's excitation level against ref), so
+// gocc = n_k(ref) g + sum_c p_c d_k(c).  The second sum takes fixed-point LDS adds (exact,
+// so their order does not matter), a few per column for strings near ref instead of one
+// per occupied orbital, and fewer of them on one address.
+constexpr int GQO = 4;
+template <bool VEC>
+__global__ void __launch_bounds__(RT) k_gram_occ(int64_t ncols, int L, const int64_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ col, const double* __restrict__ X,
+                                                 const uint64_t* __restrict__ cstr, double* __restrict__ g,
+                                                 double* __restrict__ gocc) {
+  __shared__ double sh[RT / 64][GQO];
+  __shared__ unsigned long long acc[GQO][64];
+  const int64_t r = blockIdx.x;
+  const double* xr = X + r * ncols;
+  const uint64_t ref = cstr[0];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t q1 = rowptr[r + 1];
+  for (int64_t q0 = first_above(rowptr[r], q1, col, r); q0 < q1; q0 += GQO) {
+    const int nq = (int)min<int64_t>(GQO, q1 - q0);
+    const double* xq[GQO];
+    double s[GQO];
+#pragma unroll
+    for (int u = 0; u < GQO; ++u) {
+      xq[u] = X + (int64_t)col[q0 + min(u, nq - 1)] * ncols;
+      s[u] = 0.0;
+    }
+    __syncthreads();  // the previous pass's readers of acc and sh are done
+    for (int x = threadIdx.x; x < GQO * 64; x += RT) acc[x / 64][x % 64] = 0ull;
+    __syncthreads();
+    auto column = [&](double x, const double (&y)[GQO], uint64_t occ) {
+      const uint64_t d = occ ^ ref;
+#pragma unroll
+      for (int u = 0; u < GQO; ++u)
+        if (u < nq) {
+          const double p = x * y[u];
+          s[u] += p;
+          if (p != 0.0)
+            for (uint64_t z = d; z; z &= z - 1) {
+              const int k = __builtin_ctzll(z);
+              fx_add(&acc[u][k], ((occ >> k) & 1ull) ? p : -p);
+            }
+        }
+    };
+    if (VEC) {
+      for (int64_t cc = 2 * (int64_t)threadIdx.x; cc < ncols; cc += 2 * RT) {
+        const double2 x = *reinterpret_cast<const double2*>(xr + cc);
+        double y0[GQO], y1[GQO];
+#pragma unroll
+        for (int u = 0; u < GQO; ++u) {
+          const double2 y = u < nq ? *reinterpret_cast<const double2*>(xq[u] + cc) : make_double2(0.0, 0.0);
+          y0[u] = y.x;
+          y1[u] = y.y;
+        }
+        column(x.x, y0, cstr[cc]);
+        column(x.y, y1, cstr[cc + 1]);
+      }
+    } else {
+      for (int64_t cc = threadIdx.x; cc < ncols; cc += RT) {
+        double y[GQO];
+#pragma unroll
+        for (int u = 0; u < GQO; ++u) y[u] = u < nq ? xq[u][cc] : 0.0;
+        column(xr[cc], y, cstr[cc]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GQO; ++u)
+      for (int o = 32; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, 64);
+    if (lane == 0)
+#pragma unroll
+      for (int u = 0; u < GQO; ++u) sh[wid][u] = s[u];
+    __syncthreads();
+    for (int x = threadIdx.x; x < nq * L; x += RT) {
+      const int u = x / L, k = x % L;
+      double t = 0.0;
+      for (int w = 0; w < RT / 64; ++w) t += sh[w][u];
+      if (k == 0) g[q0 + u] = t;
+      gocc[(q0 + u) * L + k] = (((ref >> k) & 1ull) ? t : 0.0) + fx_get(acc[u][k]);
     }
   }
 }
@@ -341,8 +413,9 @@ __global__ void k_keyseg(int64_t n, int nkeys, const uint32_t* __restrict__ keys
   kseg[key] = (int32_t)lo;
 }
 
-// alpha-beta: workgroup per (key, chunk of <= PCH pairs); partial[T][L*L] in
-// part[(kseg_chunk)*L2].  LDS: ket row chunk + L*L accumulators.
+// alpha-beta: workgroup w per (key, chunk of <= PCH pairs), the sorted pairs
+// [wseg[2w], wseg[2w+1]); partial[T][L*L] in part[w*L2].  LDS: ket row chunk + L*L
+// accumulators.
 constexpr int PCH = 32;
 __global__ void __launch_bounds__(RT) k_ab_rdm(int64_t nb, int L, const int32_t* __restrict__ wseg,
                                                const int32_t* __restrict__ wkey, const int32_t* __restrict__ perm,
@@ -357,7 +430,7 @@ __global__ void __launch_bounds__(RT) k_ab_rdm(int64_t nb, int L, const int32_t*
   const int LL = L * L;
   const int w = blockIdx.x;
   for (int x = threadIdx.x; x < LL; x += RT) T[x] = 0ull;
-  for (int32_t e = wseg[w]; e < wseg[w + 1]; ++e) {
+  for (int32_t e = wseg[2 * w]; e < wseg[2 * w + 1]; ++e) {
     const int32_t q = perm[e];
     const int64_t ia = rowof[q], ja = colSa[q];
     const double sa = ((excSa[q] >> 24) & 1u) ? -1.0 : 1.0;
@@ -405,7 +478,7 @@ __global__ void __launch_bounds__(RTA) k_ab_rdm_reg(int64_t nb, int L, const int
   const int LL = L * L;
   const int w = blockIdx.x;
   for (int x = threadIdx.x; x < LL; x += RTA) T[x] = 0ull;
-  for (int32_t e = wseg[w]; e < wseg[w + 1]; ++e) {
+  for (int32_t e = wseg[2 * w]; e < wseg[2 * w + 1]; ++e) {
     const int32_t q = perm[e];
     const int64_t ia = rowof[q], ja = colSa[q];
     const double sa = ((excSa[q] >> 24) & 1u) ? -1.0 : 1.0;
@@ -447,10 +520,12 @@ __global__ void k_ab_slots(int L, const int32_t* __restrict__ kfirst, const doub
   const int a = key / L, i = key % L;
   const int LL = L * L;
   const int64_t L4 = (int64_t)LL * LL;
+  const int src = a > i ? key : i * L + a;  // keys with a < i: the mirror key, (b, j) swapped
   for (int x = threadIdx.x; x < LL; x += blockDim.x) {
-    double v = 0.0;
-    for (int32_t c = kfirst[key]; c < kfirst[key + 1]; ++c) v += part[(int64_t)c * LL + x];
     const int b = x / L, j = x % L;
+    const int xs = a > i ? x : j * L + b;
+    double v = 0.0;
+    for (int32_t c = kfirst[src]; c < kfirst[src + 1]; ++c) v += part[(int64_t)c * LL + xs];
     if (a == i || b == j) continue;
     two[2 * L4 + idx4(L, a, b, i, j)] = v;
     two[1 * L4 + idx4(L, b, a, j, i)] = v;
@@ -513,8 +588,13 @@ void singles_part(const Table& t, int L, int s, const double* X, int64_t ncols, 
   PairBlock pb;
   pb.g.alloc(t.nnz);
   pb.gocc.alloc((size_t)t.nnz * L);
-  hipLaunchKernelGGL(k_gram<true>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, L, t.rowptr.p, t.col.p, X, cstr,
-                     pb.g.p, pb.gocc.p);
+  if (ncols % 2 == 0)
+    hipLaunchKernelGGL(k_gram_occ<true>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, L, t.rowptr.p, t.col.p, X,
+                       cstr, pb.g.p, pb.gocc.p);
+  else
+    hipLaunchKernelGGL(k_gram_occ<false>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, L, t.rowptr.p, t.col.p, X,
+                       cstr, pb.g.p, pb.gocc.p);
+  hipLaunchKernelGGL(k_gram_mirror, dim3((unsigned)t.n), dim3(RT), 0, st, L, t.rowptr.p, t.col.p, pb.g.p, pb.gocc.p);
   DBuf<uint32_t> keys, ks;
   DBuf<int32_t> idx, perm, kseg;
   keys.alloc(t.nnz);
@@ -537,6 +617,7 @@ void doubles_part(const Table& t, int L, int s, const double* X, int64_t ncols, 
     hipLaunchKernelGGL(k_gram_pairs<true>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, t.rowptr.p, t.col.p, X, g.p);
   else
     hipLaunchKernelGGL(k_gram_pairs<false>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, t.rowptr.p, t.col.p, X, g.p);
+  hipLaunchKernelGGL(k_gram_mirror, dim3((unsigned)t.n), dim3(RT), 0, st, L, t.rowptr.p, t.col.p, g.p, nullptr);
   DBuf<uint32_t> keys, ks;
   DBuf<int32_t> idx, perm, flag, pos, seg;
   keys.alloc(t.nnz);
@@ -631,25 +712,28 @@ void rdm_compute(const RdmPlan& P, const double* W, double* WT, double* one, dou
     std::vector<int32_t> hk(L2 + 1);
     SBDG_CHECK(hipMemcpyAsync(hk.data(), kseg.p, sizeof(int32_t) * (L2 + 1), hipMemcpyDeviceToHost, st));
     SBDG_CHECK(hipStreamSynchronize(st));
-    // split each key's run into workgroups of <= PCH pairs
+    // split each key's run into workgroups of <= PCH pairs; only keys (a, i) with a > i: the
+    // pairs of key (i, a) are the same pairs read the other way, with the same product and
+    // signs, so T_(i,a)[j,b] = T_(a,i)[b,j] (k_ab_slots reads it there)
     std::vector<int32_t> wseg, wkey, kfirst(L2 + 1);
     for (int key = 0; key < (int)L2; ++key) {
       kfirst[key] = (int32_t)wkey.size();
+      if (key / L <= key % L) continue;
       for (int32_t e = hk[key]; e < hk[key + 1]; e += PCH) {
-        wseg.push_back(e);
+        wseg.push_back(e);  // [start, end) of the workgroup's pairs
+        wseg.push_back(std::min(e + PCH, hk[key + 1]));
         wkey.push_back(key);
       }
     }
     kfirst[L2] = (int32_t)wkey.size();
-    wseg.push_back(hk[L2]);
     const int nw = (int)wkey.size();
     DBuf<int32_t> dws, dwk, dkf;
     DBuf<double> part;
-    dws.alloc(wseg.size());
+    dws.alloc(std::max<size_t>(wseg.size(), 1));
     dwk.alloc(std::max(nw, 1));
     dkf.alloc(L2 + 1);
     part.alloc((size_t)std::max(nw, 1) * L2);
-    SBDG_CHECK(hipMemcpyAsync(dws.p, wseg.data(), sizeof(int32_t) * wseg.size(), hipMemcpyHostToDevice, st));
+    if (nw) SBDG_CHECK(hipMemcpyAsync(dws.p, wseg.data(), sizeof(int32_t) * wseg.size(), hipMemcpyHostToDevice, st));
     if (nw) SBDG_CHECK(hipMemcpyAsync(dwk.p, wkey.data(), sizeof(int32_t) * nw, hipMemcpyHostToDevice, st));
     SBDG_CHECK(hipMemcpyAsync(dkf.p, kfirst.data(), sizeof(int32_t) * (L2 + 1), hipMemcpyHostToDevice, st));
     const size_t lds = sizeof(double) * (size_t)(P.ab->chunk + L2);
