@@ -260,10 +260,10 @@ def test_sigma_slice_copies_vs_oracle(L):
 
 @pytest.mark.parametrize("overlap", [1, 0])
 def test_sigma_wide_beta_set_vs_oracle(overlap):
-    """30,000 beta strings (norb=24, 5 electrons) against 12 alpha strings: the alpha-beta
-    kernel then spans several column tiles (24,576 columns each), each with its own
-    per-chunk lane order.  overlap=1 runs it on its own stream beside the spin phases,
-    overlap=0 after them on one stream."""
+    """30,000 beta strings (norb=24, 5 electrons) against 12 alpha strings: one wide column
+    tile per bra row (768 threads x 40 columns in FP64, 1024 x 40 in FP32), each chunk's
+    sums added into sigma through LDS in column order.  overlap=1 runs the alpha-beta kernel
+    on its own stream beside the spin phases, overlap=0 after them on one stream."""
     rng = np.random.default_rng(77)
     L = 24
     h1, eri = syn.synthetic_integrals(L, rng)
@@ -279,6 +279,29 @@ def test_sigma_wide_beta_set_vs_oracle(overlap):
     ms = s.bench_sigma(0, 2)
     assert ms[0] > 0 and abs(ms[1:].sum() - ms[0]) <= 1e-3 * ms[0] + 1e-3
     assert np.array_equal(s.sigma(x), sx)  # repeatable
+
+
+def test_sigma_beta_set_beyond_one_tile_vs_oracle():
+    """53,130 beta strings (norb=25, 5 electrons, the full space) against 8 alpha strings:
+    beyond the widest single tile (40,704 FP64, 49,152 FP32 columns), so the alpha-beta
+    kernel runs two column tiles per bra row, each with its own per-chunk lane order, and
+    adds each chunk's sums into sigma at the permuted columns directly.  FP64 and FP32
+    against the oracle, the kernel on its own stream and in sequence."""
+    rng = np.random.default_rng(78)
+    L = 25
+    h1, eri = syn.synthetic_integrals(L, rng)
+    case = dict(norb=L, core=0.5, h1=h1, eri=eri, astr=syn.sample_strings(L, 5, 8, rng),
+                bstr=syn.full_space(L, 5))
+    p = oproblem(case)
+    x = rng.normal(size=len(case["astr"]) * len(case["bstr"]))
+    ox = p.sigma(x)
+    for overlap in (1, 0):
+        s = solver(case, overlap=overlap)
+        sx = s.sigma(x)
+        assert np.abs(sx - ox).max() <= 1e-12 * np.abs(ox).max(), overlap
+        assert np.abs(s.sigma(x, precision=1) - ox).max() <= 2e-5 * np.abs(ox).max(), overlap
+        assert np.array_equal(s.sigma(x), sx)  # repeatable
+        s.close()
 
 
 def test_core_workspace_cap():
