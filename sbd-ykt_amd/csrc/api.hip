@@ -808,6 +808,10 @@ void rdm_or_density(sbdg_ctx* c) {
     P.sb = &c->Sb;
     P.db = &c->Db;
     P.ab = &c->ab64;
+    if (c->opt.doubles != 1) {  // the core tables exist (built for sigma, used or not)
+      P.ca = &c->Ca;
+      P.cb = &c->Cb;
+    }
     c->T.alloc(na * nb);  // the transposed vector
     rdm_compute(P, c->W.p, c->T.p, c->d_one.p, c->d_two.p, c->st);
     c->have_rdm = true;

@@ -82,28 +82,122 @@ __global__ void k_gram_mirror(int L, const int64_t* __restrict__ rowptr, const i
   }
 }
 
-// g[q] = <X[r,:], X[col[q],:]> for the pairs of row r, GQ pairs per pass: row r is read
-// once per pass for all of them and their sums are reduced together (one barrier pair per
-// pass instead of per pair).  Each sum's order is fixed (a thread's columns in order, then
+// entry of partner c in row r of an excitation table (partners ascending); -1 if absent
+__device__ inline int64_t find_entry(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                     int64_t r, int64_t c) {
+  const int64_t e = first_above(rowptr[r], rowptr[r + 1], col, c - 1);
+  return e < rowptr[r + 1] && col[e] == c ? e : -1;
+}
+
+// Gram products of the members of one large core of the same-spin doubles (members
+// x = core + {A, B}, as sigma groups them): every member row pairs with every other, so a
+// workgroup takes a block pair (bi, bj) of one core's members, BM rows each, and streams
+// them through LDS in tiles of GT columns once for all BM^2 products (thread (ti, tj) owns
+// a TM x TM sub-block).  A member pair whose strings differ in four orbitals is a double
+// pair, and that pair shares no other core: its sum goes to both entries of the
+// excitation table, (x, y) and (y, x), and marks them done.  Fixed summation order.
+template <int BM>
+__global__ void __launch_bounds__(RT) k_core_gram(int64_t ncols, const int4* __restrict__ items,
+                                                  const int32_t* __restrict__ cstart,
+                                                  const int32_t* __restrict__ crow, const uint64_t* __restrict__ str,
+                                                  const int64_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ col, const double* __restrict__ X,
+                                                  double* __restrict__ g, unsigned char* __restrict__ done) {
+  constexpr int TM = BM / 16, GT = 2048 / BM;
+  __shared__ double A[GT][BM + 1], B[GT][BM + 1];
+  const int4 it = items[blockIdx.x];  // core, block bi, block bj
+  const int32_t s0 = cstart[it.x], m = cstart[it.x + 1] - s0;
+  const int i0 = it.y * BM, j0 = it.z * BM;
+  const bool same = it.y == it.z;
+  const int ti = threadIdx.x / 16, tj = threadIdx.x % 16;
+  double acc[TM][TM];
+#pragma unroll
+  for (int u = 0; u < TM; ++u)
+#pragma unroll
+    for (int v = 0; v < TM; ++v) acc[u][v] = 0.0;
+  for (int64_t c0 = 0; c0 < ncols; c0 += GT) {
+    __syncthreads();  // the previous tile's readers are done
+    for (int x = threadIdx.x; x < BM * GT; x += RT) {
+      const int r = x / GT, cc = x % GT;
+      const int64_t c = c0 + cc;
+      A[cc][r] = (i0 + r < m && c < ncols) ? X[(int64_t)crow[s0 + i0 + r] * ncols + c] : 0.0;
+      if (!same) B[cc][r] = (j0 + r < m && c < ncols) ? X[(int64_t)crow[s0 + j0 + r] * ncols + c] : 0.0;
+    }
+    __syncthreads();
+    const double(*Bt)[BM + 1] = same ? A : B;
+    for (int cc = 0; cc < GT; ++cc) {
+      double a[TM], b[TM];
+#pragma unroll
+      for (int u = 0; u < TM; ++u) a[u] = A[cc][ti * TM + u];
+#pragma unroll
+      for (int v = 0; v < TM; ++v) b[v] = Bt[cc][tj * TM + v];
+#pragma unroll
+      for (int u = 0; u < TM; ++u)
+#pragma unroll
+        for (int v = 0; v < TM; ++v) acc[u][v] += a[u] * b[v];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < TM; ++u)
+#pragma unroll
+    for (int v = 0; v < TM; ++v) {
+      const int i = i0 + ti * TM + u, j = j0 + tj * TM + v;
+      if (i >= m || j >= m || i >= j) continue;
+      const int64_t ri = crow[s0 + i], rj = crow[s0 + j];
+      if (__popcll(str[ri] ^ str[rj]) != 4) continue;  // pairs sharing an orbital: a single
+      const int64_t q1 = find_entry(rowptr, col, ri, rj), q2 = find_entry(rowptr, col, rj, ri);
+      if (q1 < 0 || q2 < 0) continue;
+      g[q1] = acc[u][v];
+      g[q2] = acc[u][v];
+      done[q1] = 1;
+      done[q2] = 1;
+    }
+}
+
+// the entries above the diagonal that no core product formed, as a list per row
+__global__ void k_pair_flags(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                             const unsigned char* __restrict__ done, int32_t* __restrict__ flag) {
+  const int64_t r = blockIdx.x;
+  for (int64_t q = rowptr[r] + threadIdx.x; q < rowptr[r + 1]; q += blockDim.x)
+    flag[q] = (col[q] > r && !done[q]) ? 1 : 0;
+}
+__global__ void k_pair_list(int64_t nnz, const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                            int32_t* __restrict__ qlist) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nnz && flag[q]) qlist[pos[q]] = (int32_t)q;
+}
+__global__ void k_pair_lptr(int64_t n, int64_t nnz, const int64_t* __restrict__ rowptr,
+                            const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                            int64_t* __restrict__ lptr) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  const int64_t q = rowptr[r];
+  lptr[r] = q < nnz ? pos[q] : (nnz ? (int64_t)pos[nnz - 1] + flag[nnz - 1] : 0);
+}
+
+// g[q] = <X[r,:], X[col[q],:]> for the pairs q = qlist[e], e in [lptr[r], lptr[r+1]), of
+// row r, GQ pairs per pass: row r is read once per pass for all of them and their sums are
+// reduced together (one barrier pair per pass instead of per pair).  Each sum's order is fixed (a thread's columns in order, then
 // a shuffle tree, then the waves in order), so g does not depend on timing.  VEC: rows of
 // an even length, read two columns at a time.
 constexpr int GQ = 8;
 template <bool VEC>
-__global__ void __launch_bounds__(RT) k_gram_pairs(int64_t ncols, const int64_t* __restrict__ rowptr,
+__global__ void __launch_bounds__(RT) k_gram_pairs(int64_t ncols, const int64_t* __restrict__ lptr,
+                                                   const int32_t* __restrict__ qlist,
                                                    const int32_t* __restrict__ col, const double* __restrict__ X,
                                                    double* __restrict__ g) {
   __shared__ double sh[RT / 64][GQ];
   const int64_t r = blockIdx.x;
   const double* xr = X + r * ncols;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int64_t q1 = rowptr[r + 1];
-  for (int64_t q0 = first_above(rowptr[r], q1, col, r); q0 < q1; q0 += GQ) {
+  const int64_t q1 = lptr[r + 1];
+  for (int64_t q0 = lptr[r]; q0 < q1; q0 += GQ) {
     const int nq = (int)min<int64_t>(GQ, q1 - q0);
     const double* xq[GQ];
     double s[GQ];
 #pragma unroll
     for (int u = 0; u < GQ; ++u) {
-      xq[u] = X + (int64_t)col[q0 + min(u, nq - 1)] * ncols;
+      xq[u] = X + (int64_t)col[qlist[q0 + min(u, nq - 1)]] * ncols;
       s[u] = 0.0;
     }
     if (VEC) {
@@ -136,7 +230,7 @@ __global__ void __launch_bounds__(RT) k_gram_pairs(int64_t ncols, const int64_t*
     if ((int)threadIdx.x < nq) {
       double t = 0.0;
       for (int w = 0; w < RT / 64; ++w) t += sh[w][threadIdx.x];
-      g[q0 + threadIdx.x] = t;
+      g[qlist[q0 + threadIdx.x]] = t;
     }
   }
 }
@@ -609,15 +703,76 @@ void singles_part(const Table& t, int L, int s, const double* X, int64_t ncols, 
   SBDG_CHECK(hipStreamSynchronize(st));
 }
 
-void doubles_part(const Table& t, int L, int s, const double* X, int64_t ncols, double* two, hipStream_t st) {
+// Row-pair sums of the doubles table t: by core block for the cores of ct (the larger
+// cores, when sigma built them), by pair for the rest of the entries above the diagonal,
+// then mirrored to the entries below it.
+void doubles_gram(const Table& t, const CoreTable* ct, const uint64_t* strs, const double* X, int64_t ncols,
+                  double* g, hipStream_t st) {
+  DBuf<unsigned char> done;
+  done.alloc(t.nnz);
+  SBDG_CHECK(hipMemsetAsync(done.p, 0, t.nnz, st));
+  if (ct && ct->ncore > 0) {
+    std::vector<int32_t> hs(ct->ncore + 1);
+    SBDG_CHECK(hipMemcpyAsync(hs.data(), ct->start.p, sizeof(int32_t) * hs.size(), hipMemcpyDeviceToHost, st));
+    SBDG_CHECK(hipStreamSynchronize(st));
+    std::vector<int4> items[3];  // blocks of 16, 32, 64 members
+    for (int64_t k = 0; k < ct->ncore; ++k) {
+      const int m = hs[k + 1] - hs[k];
+      if (m < 2) continue;
+      const int w = m <= 16 ? 0 : (m <= 32 ? 1 : 2), bm = 16 << w, nbk = (m + bm - 1) / bm;
+      for (int bi = 0; bi < nbk; ++bi)
+        for (int bj = bi; bj < nbk; ++bj) items[w].push_back(make_int4((int)k, bi, bj, 0));
+    }
+    DBuf<int4> ditems[3];
+    for (int w = 0; w < 3; ++w) {
+      if (items[w].empty()) continue;
+      ditems[w].alloc(items[w].size());
+      SBDG_CHECK(hipMemcpyAsync(ditems[w].p, items[w].data(), sizeof(int4) * items[w].size(), hipMemcpyHostToDevice,
+                                st));
+    }
+#define SBDG_CG(W, BM)                                                                                      \
+  if (!items[W].empty())                                                                                    \
+  hipLaunchKernelGGL(k_core_gram<BM>, dim3((unsigned)items[W].size()), dim3(RT), 0, st, ncols, ditems[W].p, \
+                     ct->start.p, ct->row.p, strs, t.rowptr.p, t.col.p, X, g, done.p)
+    SBDG_CG(0, 16);
+    SBDG_CG(1, 32);
+    SBDG_CG(2, 64);
+#undef SBDG_CG
+    SBDG_CHECK(hipGetLastError());
+    SBDG_CHECK(hipStreamSynchronize(st));  // the item lists on the host stay alive until here
+  }
+  DBuf<int32_t> flag, pos, qlist;
+  DBuf<int64_t> lptr;
+  flag.alloc(t.nnz);
+  pos.alloc(t.nnz);
+  qlist.alloc(t.nnz);
+  lptr.alloc(t.n + 1);
+  hipLaunchKernelGGL(k_pair_flags, dim3((unsigned)t.n), dim3(RT), 0, st, t.rowptr.p, t.col.p, done.p, flag.p);
+  size_t tmp = 0;
+  SBDG_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flag.p, pos.p, (int)t.nnz, st));
+  DBuf<unsigned char> tb;
+  tb.alloc(tmp + 1);
+  SBDG_CHECK(hipcub::DeviceScan::ExclusiveSum(tb.p, tmp, flag.p, pos.p, (int)t.nnz, st));
+  hipLaunchKernelGGL(k_pair_list, dim3(nb_for(t.nnz, 256)), dim3(256), 0, st, t.nnz, flag.p, pos.p, qlist.p);
+  hipLaunchKernelGGL(k_pair_lptr, dim3(nb_for(t.n + 1, 256)), dim3(256), 0, st, t.n, t.nnz, t.rowptr.p, flag.p, pos.p,
+                     lptr.p);
+  if (ncols % 2 == 0)
+    hipLaunchKernelGGL(k_gram_pairs<true>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, lptr.p, qlist.p, t.col.p, X,
+                       g);
+  else
+    hipLaunchKernelGGL(k_gram_pairs<false>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, lptr.p, qlist.p, t.col.p,
+                       X, g);
+  hipLaunchKernelGGL(k_gram_mirror, dim3((unsigned)t.n), dim3(RT), 0, st, 0, t.rowptr.p, t.col.p, g, nullptr);
+  SBDG_CHECK(hipGetLastError());
+  SBDG_CHECK(hipStreamSynchronize(st));
+}
+
+void doubles_part(const Table& t, int L, int s, const double* X, int64_t ncols, const CoreTable* ct,
+                  const uint64_t* strs, double* two, hipStream_t st) {
   if (t.nnz == 0) return;
   DBuf<double> g;
   g.alloc(t.nnz);
-  if (ncols % 2 == 0)
-    hipLaunchKernelGGL(k_gram_pairs<true>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, t.rowptr.p, t.col.p, X, g.p);
-  else
-    hipLaunchKernelGGL(k_gram_pairs<false>, dim3((unsigned)t.n), dim3(RT), 0, st, ncols, t.rowptr.p, t.col.p, X, g.p);
-  hipLaunchKernelGGL(k_gram_mirror, dim3((unsigned)t.n), dim3(RT), 0, st, L, t.rowptr.p, t.col.p, g.p, nullptr);
+  doubles_gram(t, ct, strs, X, ncols, g.p, st);
   DBuf<uint32_t> keys, ks;
   DBuf<int32_t> idx, perm, flag, pos, seg;
   keys.alloc(t.nnz);
@@ -689,10 +844,10 @@ void rdm_compute(const RdmPlan& P, const double* W, double* WT, double* one, dou
   }
   // alpha pairs (beta spectators from the beta strings of the columns)
   singles_part(*P.sa, L, 0, W, nb, P.a, P.b, one, two, st);
-  doubles_part(*P.da, L, 0, W, nb, two, st);
+  doubles_part(*P.da, L, 0, W, nb, P.ca, P.a, two, st);
   // beta pairs on the transposed vector (alpha spectators from the alpha strings)
   singles_part(*P.sb, L, 1, WT, na, P.b, P.a, one, two, st);
-  doubles_part(*P.db, L, 1, WT, na, two, st);
+  doubles_part(*P.db, L, 1, WT, na, P.cb, P.b, two, st);
 
   // alpha-beta
   const Table& sa = *P.sa;

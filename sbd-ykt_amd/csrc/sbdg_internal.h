@@ -327,6 +327,8 @@ struct RdmPlan {
   const Table* sb = nullptr;
   const Table* db = nullptr;
   const AbTable* ab = nullptr;
+  const CoreTable* ca = nullptr;  // larger cores of the same-spin doubles (row-pair sums by block), or null
+  const CoreTable* cb = nullptr;
 };
 // one[2][L^2], two[4][L^4] in the reference layout (correlation.h:194-444) from the
 // separated-convention vector W; WT: scratch of na*nb doubles (W transposed).  Bitwise
