@@ -34,6 +34,10 @@ struct V16<float> {
 };
 
 constexpr int TS = 256;  // threads of the row-streaming product
+#ifndef SBDG_SPMM_RW
+#define SBDG_SPMM_RW 4
+#endif
+constexpr int SPMM_RW = SBDG_SPMM_RW;  // rows per workgroup of the row product
 
 // Column range [cb, ce) of one row product:
 //   out[r, c] = (DIAG ? hii*X : 0)[r, c] + sum_q (val[q] + sgn_q J[p_q][c]) X[col[q], c]
@@ -54,14 +58,17 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
                                              const T* __restrict__ Z, int64_t zcols) {
   constexpr int VN = V16<T>::n;
   using VT = typename V16<T>::t;
-  const int64_t ntl = (ce - cb + TS * VN - 1) / (TS * VN);
+  constexpr int RT = TS / SPMM_RW;  // threads per row
+  const int64_t ntl = (ce - cb + RT * VN - 1) / (RT * VN);
   const int64_t items = ntl * nrows;
-  const int64_t per = (items + 7) / 8;
+  const int64_t nwg = (items + SPMM_RW - 1) / SPMM_RW;
+  const int64_t per = (nwg + 7) / 8;
   const int64_t bid = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;  // each XCD: a contiguous run of rows
-  if (bid >= items) return;
-  const int64_t tile = bid / nrows;
-  const int64_t r = bid - tile * nrows;
-  const int64_t c0 = cb + tile * (int64_t)(TS * VN) + (int64_t)threadIdx.x * VN;
+  const int64_t item = bid * SPMM_RW + threadIdx.x / RT;
+  if (item >= items) return;  // no barrier in this kernel: a row group may leave alone
+  const int64_t tile = item / nrows;
+  const int64_t r = item - tile * nrows;
+  const int64_t c0 = cb + tile * (int64_t)(RT * VN) + (int64_t)(threadIdx.x % RT) * VN;
   if (c0 >= ce) return;
   T acc[VN];
   const bool full = VECTOR && (c0 + VN <= ce);
@@ -570,8 +577,8 @@ void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, c
   if (ce < 0) ce = ncols;
   if (nrows == 0 || ce <= cb) return;
   constexpr int VN = V16<T>::n;
-  const int64_t ntiles = (ce - cb + TS * VN - 1) / (TS * VN);
-  const int64_t nblk = (ntiles * nrows + 7) / 8 * 8;
+  const int64_t ntiles = (ce - cb + TS / SPMM_RW * VN - 1) / (TS / SPMM_RW * VN);
+  const int64_t nblk = ((ntiles * nrows + SPMM_RW - 1) / SPMM_RW + 7) / 8 * 8;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
   const T* val = nullptr;
   if constexpr (sizeof(T) == 8) val = reinterpret_cast<const T*>(tb.val.p);
