@@ -98,3 +98,23 @@ def core_split(strs, kmin=10):
         else:
             direct += sum(1 for j, x in enumerate(keep) for t, y in enumerate(keep) if t != j and (x & y) == 0)
     return nmem, direct
+
+
+def rdm_invariants(one, two, na_e, nb_e, L, tol):
+    """tr gamma^s = N_s; gamma Hermitian; sum_j Gamma^{st}[i j k j] = (N_t - delta_st)
+    gamma^s[i k] for every block (the library's mixed blocks are the physical ones)."""
+    assert abs(one[0].reshape(L, L).trace() - na_e) < tol
+    assert abs(one[1].reshape(L, L).trace() - nb_e) < tol
+    for s in range(2):
+        G = one[s].reshape(L, L)
+        assert np.abs(G - G.T).max() < tol
+    for s in range(2):
+        for t in range(2):
+            G = two[s + 2 * t].reshape(L, L, L, L, order="F")
+            Nt = (na_e, nb_e)[t] - (1 if s == t else 0)
+            pt = np.einsum("ijkj->ik", G)
+            assert np.abs(pt - Nt * one[s].reshape(L, L, order="F")).max() < tol, (s, t)
+    # Hermiticity of the 2-RDM: Gamma[i j k l] = Gamma[k l i j] within each block
+    for b in range(4):
+        G = two[b].reshape(L, L, L, L, order="F")
+        assert np.abs(G - G.transpose(2, 3, 0, 1)).max() < tol, b

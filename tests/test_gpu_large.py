@@ -26,7 +26,7 @@ import pytest
 import oracle
 import sbdg
 from conftest import load_golden
-from helpers import SMALL_CASES, energy_from_rdm, rdm_close, small_case, workload_small
+from helpers import SMALL_CASES, energy_from_rdm, rdm_close, rdm_invariants, small_case, workload_small
 from sbdg import synthetic as syn
 
 pytestmark = pytest.mark.gpu
@@ -42,26 +42,6 @@ def oproblem(case):
 
 def as_case(wl):
     return dict(norb=wl.norb, core=wl.core, h1=wl.h1, eri=wl.eri, astr=wl.astr, bstr=wl.bstr)
-
-
-def rdm_invariants(one, two, na_e, nb_e, L, tol):
-    """tr gamma^s = N_s; gamma Hermitian; sum_j Gamma^{st}[i j k j] = (N_t - delta_st)
-    gamma^s[i k] for every block (the library's mixed blocks are the physical ones)."""
-    assert abs(one[0].reshape(L, L).trace() - na_e) < tol
-    assert abs(one[1].reshape(L, L).trace() - nb_e) < tol
-    for s in range(2):
-        G = one[s].reshape(L, L)
-        assert np.abs(G - G.T).max() < tol
-    for s in range(2):
-        for t in range(2):
-            G = two[s + 2 * t].reshape(L, L, L, L, order="F")
-            Nt = (na_e, nb_e)[t] - (1 if s == t else 0)
-            pt = np.einsum("ijkj->ik", G)
-            assert np.abs(pt - Nt * one[s].reshape(L, L, order="F")).max() < tol, (s, t)
-    # Hermiticity of the 2-RDM: Gamma[i j k l] = Gamma[k l i j] within each block
-    for b in range(4):
-        G = two[b].reshape(L, L, L, L, order="F")
-        assert np.abs(G - G.transpose(2, 3, 0, 1)).max() < tol, b
 
 
 # ---------------------------------------------------------------- norb=58, 12+12

@@ -12,8 +12,8 @@ import bruteforce as bf
 import oracle
 import sbdg
 from conftest import load_golden
-from helpers import (SMALL_CASES, core_split, dataset, energy_from_rdm, rdm_close, small_case, tables_from_ref,
-                     workload_small)
+from helpers import (SMALL_CASES, core_split, dataset, energy_from_rdm, rdm_close, rdm_invariants, small_case,
+                     tables_from_ref, workload_small)
 from sbdg import fcidump as fio
 from sbdg import synthetic as syn
 
@@ -301,6 +301,35 @@ def test_sigma_beta_set_beyond_one_tile_vs_oracle():
         assert np.abs(sx - ox).max() <= 1e-12 * np.abs(ox).max(), overlap
         assert np.abs(s.sigma(x, precision=1) - ox).max() <= 2e-5 * np.abs(ox).max(), overlap
         assert np.array_equal(s.sigma(x), sx)  # repeatable
+        s.close()
+
+
+def test_rdm_beta_set_beyond_register_rows_vs_oracle():
+    """The 1- and 2-RDMs of a seeded unit vector over 53,130 beta strings (norb=25, 5
+    electrons, the full space) and 6 alpha strings against the oracle (the mixed-spin blocks
+    as their sum: the oracle keeps the reference's block labels), each block's partial
+    traces and Hermiticity.  At more than 40 bra values per thread (40,960 columns) the
+    alpha-beta part leaves the register form for the kernel that re-reads the bra row per
+    ket chunk; the same-spin doubles take the excitation-table form (doubles=1) and the
+    core-block form (doubles=2)."""
+    rng = np.random.default_rng(79)
+    L = 25
+    h1, eri = syn.synthetic_integrals(L, rng)
+    case = dict(norb=L, core=0.5, h1=h1, eri=eri, astr=syn.sample_strings(L, 5, 6, rng),
+                bstr=syn.full_space(L, 5))
+    w = rng.normal(size=len(case["astr"]) * len(case["bstr"]))
+    w /= np.linalg.norm(w)
+    one_o, two_o = oproblem(case).rdm(w)
+    for doubles in (1, 2):
+        s = solver(case, doubles=doubles, max_it=1, max_nb=2, eps=1e10, init=2, do_rdm=1)
+        s.set_vector(w)
+        s.run()
+        assert (s.stats()["core_members"][1] > 0) == (doubles == 2)
+        one, two = s.rdm()
+        rdm_close(one, two, one_o, two_o, 1e-11, mixed_sum=True)
+        rdm_invariants(one, two, 5, 5, L, 1e-11)
+        one2, two2 = s.rdm(compute=True)
+        assert np.array_equal(one, one2) and np.array_equal(two, two2), doubles  # same bits
         s.close()
 
 
