@@ -34,20 +34,28 @@ struct V16<float> {
 };
 
 constexpr int TS = 256;  // threads of the row-streaming product
-#ifndef SBDG_SPMM_RW
-#define SBDG_SPMM_RW 4
-#endif
-constexpr int SPMM_RW = SBDG_SPMM_RW;  // rows per workgroup of the row product
+constexpr int SPMM_U = 4;  // table entries gathered per step of the row product
+
+// rows per workgroup of the row product: FP64 one row of 512 columns, FP32 four rows (one
+// wave each) of 256 columns
+template <class T>
+constexpr int spmm_rows() {
+  return sizeof(T) == 8 ? 1 : 4;
+}
 
 // Column range [cb, ce) of one row product:
 //   out[r, c] = (DIAG ? hii*X : 0)[r, c] + sum_q (val[q] + sgn_q J[p_q][c]) X[col[q], c]
 //             + (ZG ? sum_{s in slots(r)} Z[slot[s], c - cb] : 0)
 // jai[q] = +-(p_q) selects the row of the column strings' occupation table J that carries
 // the other-spin occupation part of a single (row 0: zero, for doubles).  Z holds the
-// core-grouped doubles of the same column range (k_core_prod).  Workgroups are dealt to the
-// 8 XCDs in turn and workgroup L takes item (L % 8) * per + L / 8 (per = ceil(items / 8)):
-// each XCD walks a contiguous run of rows, whose strings share many excitation partners,
-// so the rows of X they read meet in that XCD's L2.
+// core-grouped doubles of the same column range (k_core_prod).  A work item is one row of
+// one column tile; tiles are outer, rows inner.  Workgroups take spmm_rows consecutive
+// items, are dealt to the 8 XCDs in turn, and workgroup L takes items from
+// (L % 8) * per + L / 8 (per = ceil(workgroups / 8)): each XCD walks a contiguous run of
+// rows, whose strings share many excitation partners, so the rows of X they read meet in
+// that XCD's L2.  Measured on one GPU against one 1,024-column row per FP32 workgroup, four
+// rows of 256 columns take 3-4 % less time per pass; for FP64 four rows of 128 columns take
+// 4-11 % more than one row of 512, and 2 or 8 rows, or 8 entries per step, were slower.
 template <class T, bool DIAG, bool VECTOR, bool ZG>
 __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64_t cb, int64_t ce,
                                              const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -58,17 +66,17 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
                                              const T* __restrict__ Z, int64_t zcols) {
   constexpr int VN = V16<T>::n;
   using VT = typename V16<T>::t;
-  constexpr int RT = TS / SPMM_RW;  // threads per row
+  constexpr int RW = spmm_rows<T>();
+  constexpr int RT = TS / RW;  // threads per row
   const int64_t ntl = (ce - cb + RT * VN - 1) / (RT * VN);
   const int64_t items = ntl * nrows;
-  const int64_t nwg = (items + SPMM_RW - 1) / SPMM_RW;
-  const int64_t per = (nwg + 7) / 8;
+  const int64_t per = ((items + RW - 1) / RW + 7) / 8;
   const int64_t bid = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;  // each XCD: a contiguous run of rows
-  const int64_t item = bid * SPMM_RW + threadIdx.x / RT;
+  const int64_t item = bid * RW + (RW == 1 ? 0 : threadIdx.x / RT);
   if (item >= items) return;  // no barrier in this kernel: a row group may leave alone
   const int64_t tile = item / nrows;
   const int64_t r = item - tile * nrows;
-  const int64_t c0 = cb + tile * (int64_t)(RT * VN) + (int64_t)(threadIdx.x % RT) * VN;
+  const int64_t c0 = cb + tile * (int64_t)(RT * VN) + (int64_t)(RW == 1 ? threadIdx.x : threadIdx.x % RT) * VN;
   if (c0 >= ce) return;
   T acc[VN];
   const bool full = VECTOR && (c0 + VN <= ce);
@@ -92,11 +100,11 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
   int64_t q = rowptr[r];
   const int64_t q1 = rowptr[r + 1];
   if (full) {
-    for (; q + 4 <= q1; q += 4) {
-      int32_t j[4];
-      T v[4], sg[4], x[4][VN], jv[4][VN];
+    for (; q + SPMM_U <= q1; q += SPMM_U) {
+      int32_t j[SPMM_U];
+      T v[SPMM_U], sg[SPMM_U], x[SPMM_U][VN], jv[SPMM_U][VN];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < SPMM_U; ++u) {
         j[u] = col[q + u];
         v[u] = val[q + u];
         const int32_t p = jai[q + u];
@@ -105,7 +113,7 @@ __global__ void __launch_bounds__(TS) k_spmm(int64_t nrows, int64_t ncols, int64
         *reinterpret_cast<VT*>(jv[u]) = *reinterpret_cast<const VT*>(J + (int64_t)(p < 0 ? -p : p) * ncols + c0);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < SPMM_U; ++u)
 #pragma unroll
         for (int k = 0; k < VN; ++k) acc[k] += (v[u] + sg[u] * jv[u][k]) * x[u][k];
     }
@@ -577,8 +585,9 @@ void launch_spmm(bool diag, int64_t nrows, int64_t ncols, const SpmmTable& tb, c
   if (ce < 0) ce = ncols;
   if (nrows == 0 || ce <= cb) return;
   constexpr int VN = V16<T>::n;
-  const int64_t ntiles = (ce - cb + TS / SPMM_RW * VN - 1) / (TS / SPMM_RW * VN);
-  const int64_t nblk = ((ntiles * nrows + SPMM_RW - 1) / SPMM_RW + 7) / 8 * 8;
+  constexpr int RW = spmm_rows<T>();
+  const int64_t ntiles = (ce - cb + TS / RW * VN - 1) / (TS / RW * VN);
+  const int64_t nblk = ((ntiles * nrows + RW - 1) / RW + 7) / 8 * 8;
   if (nblk > (int64_t)INT32_MAX) throw Error(SBDG_ERR_LIMIT, "grid too large");
   const T* val = nullptr;
   if constexpr (sizeof(T) == 8) val = reinterpret_cast<const T*>(tb.val.p);
