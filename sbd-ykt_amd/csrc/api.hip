@@ -349,8 +349,7 @@ int64_t build_bytes(const sbdg_options& o, int L, int64_t na, int64_t nb, const 
   const int64_t Sb = nnz[2];
   const int64_t ach = rdm_chunk(nb, L);
   b += 4 * Sb + 4 * (nb * ((nb + ach - 1) / ach + 1) + 1);
-  const int64_t nch = (nb + std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32) - 1) /
-                      std::max<int64_t>(256, (int64_t)(LDS_BUDGET / 16) - 2 * L2 - 32);
+  const int64_t nch = (nb + choose_chunk(nb, L) - 1) / choose_chunk(nb, L);
   const int64_t groups = (nb + 63) / 64 + 1024;  // tiles round the lane positions up
   int nt64, m64, nt32, m32;
   ab_geometry(nb, 8, nt64, m64);

@@ -134,14 +134,13 @@ struct AbTable {
 // at a time, with the next four rows in flight across group boundaries.
 // Entry: bits 0-17 LDS byte address of the ket vector (16 * (kbase + column offset inside
 // the chunk); offset `chunk` is a zero vector: padding), bits 18-29 the slice vector of
-// orbital pair b*L+j (b*L+j, or sb2 + b*L+j in the second copy of the slices), bit 31
-// sign.  Within a lane the entries are ordered, and each takes the slice copy, so that
-// lanes read distinct LDS slots where possible.
+// orbital pair (b, j) (ab_pair(b, j)), bit 31 sign.  Within a lane the entries are ordered
+// so that lanes read distinct LDS slots where possible.
 struct EllTable {
   int nch = 0;
   int nt = 0, m = 0;        // alpha-beta launch: threads per workgroup, columns per thread
   int64_t chunk = 0, groups = 0, ntiles = 0;
-  int64_t sb2 = 0, kbase = 0;  // second slice copy (0: one copy), first ket vector (16-byte units)
+  int64_t kbase = 0;        // first ket vector (16-byte units): after the slice vectors
   DBuf<int64_t> soff;       // ntiles*(nt/64)*nch + 1: start of each wave's entry stream per chunk
   DBuf<int32_t> nq;         // ntiles*(nt/64)*nch*m: four-row batches of each group of the stream
   DBuf<uint32_t> ent;
@@ -250,14 +249,16 @@ void sigma_apply(const SigmaPlan& P, const T* c, T* sigma, const SigmaScratch<T>
                  hipEvent_t* ev = nullptr);
 size_t ab_lds_bytes(int64_t kbase, int64_t chunk);
 
-// The alpha-beta kernel's LDS in 16-byte vectors: L*L integral slice vectors, a second copy
-// of them starting 8 slots (128 bytes) further along the 256-byte bank row when an entry's
-// 12-bit slice field reaches it (the entry schedule then picks, per entry, the copy whose
-// bank is free), then the ket vectors of one chunk.
-inline int64_t ab_second_slice(int L) {
-  const int64_t LL = (int64_t)L * L;
-  const int64_t b = LL + (24 - LL % 16) % 16;  // b % 16 == 8
-  return b + LL <= 4096 ? b : 0;
+// The alpha-beta kernel's LDS in 16-byte vectors: one integral slice vector per orbital pair
+// b > j at ab_pair(b, j) = b(b-1)/2 + j (a single moves an electron between two different
+// orbitals, and (ai|bj) = (ai|jb) for the 8-fold symmetric integrals the library takes), at
+// least one vector; then the ket vectors of one chunk.
+inline int64_t ab_pairs(int L) {
+  return std::max<int64_t>(1, (int64_t)L * (L - 1) / 2);
+}
+inline uint32_t ab_pair(uint32_t b, uint32_t j) {
+  const uint32_t h = b > j ? b : j, l = b > j ? j : b;
+  return h * (h - 1) / 2 + l;
 }
 // Ket chunk of the RDM's alpha-beta kernel: the chunk's FP64 values and the L*L
 // accumulators of one orbital pair fill a CU's 160 KiB of LDS.
@@ -266,8 +267,7 @@ inline int64_t rdm_chunk(int64_t nb, int L) {
   return std::min<int64_t>(std::max<int64_t>(nb, 1), ch);
 }
 inline int64_t ab_slice_span(int L) {
-  const int64_t b = ab_second_slice(L);
-  return b ? b + (int64_t)L * L : (int64_t)L * L;
+  return ab_pairs(L);
 }
 
 // vector kernels (linalg.hip)

@@ -325,8 +325,8 @@ __global__ void __launch_bounds__(256) k_transpose(int64_t R, int64_t C, const T
 // ja in S(ia) are taken G at a time (G = 2 in FP64, 4 in FP32: 16 bytes either way):
 // the G (ai|..) slices and the G rows' current column chunk sit in LDS interleaved, so one
 // 16-byte LDS read serves all G rows, and every ELL entry read from the L2-resident table
-// serves them too.  LDS: [L*L slice G-vectors] ([a second copy at sb2]) [chunk + 1 ket
-// G-vectors from kbase, the last one zero].
+// serves them too.  LDS: [slice G-vectors of the pairs b > j] [chunk + 1 ket G-vectors
+// from kbase, the last one zero].
 template <class T>
 struct AbRows {
   static constexpr int G = 16 / (int)sizeof(T);
@@ -365,7 +365,7 @@ __device__ inline void ab_batch(const uint32_t (&p)[4], T (&s)[AbRows<T>::G]) {
   RV v[4], x[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    v[u] = lds_at<RV>((p[u] >> 14) & 0xFFF0u);  // slice vector of pair b*L+j
+    v[u] = lds_at<RV>((p[u] >> 14) & 0xFFF0u);  // slice vector of pair (b, j)
     x[u] = lds_at<RV>(p[u] & 0x3FFFFu);         // ket vector
   }
 #pragma unroll
@@ -398,19 +398,22 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
                                              const int64_t* __restrict__ soff, const int32_t* __restrict__ nq,
                                              const uint32_t* __restrict__ ent,
                                              const int32_t* __restrict__ colperm, int nch, int64_t chunk,
-                                             int64_t sb2, int64_t kbase,
+                                             int64_t kbase,
                                              int64_t groups, const T* __restrict__ eri, const T* __restrict__ c,
                                              T* __restrict__ sigma, int assign) {
   using R = AbRows<T>;
   using RV = typename R::V;
   constexpr int G = R::G;
-  // positions per thread per staging round: two 16-byte vectors each (slice and ket), 8*SR
-  // VGPRs beside the accumulators (eight rounds' worth only where they fit without spills)
+  // 16-byte vectors per thread per staging round: SR ket vectors (4*SR VGPRs beside the
+  // accumulators; eight only where they fit without spills) and SV slice vectors (at most
+  // 2,016 pairs: one round at 1,024 threads)
   constexpr int SR = (sizeof(T) == 8 && M <= 20) ? 8 : 4;
+  constexpr int SV = 2;
   extern __shared__ __align__(16) unsigned char smem[];
-  RV* Vv = reinterpret_cast<RV*>(smem);  // L*L slice G-vectors (twice when sb2 > 0)
+  RV* Vv = reinterpret_cast<RV*>(smem);  // slice G-vectors of the pairs b > j
   RV* cbv = Vv + kbase;                   // chunk + 1 G-vectors (the last one stays zero: padding)
   const int64_t LL = (int64_t)L * L;
+  const int NP = L * (L - 1) / 2, NPS = max(NP, 1);  // slice vectors: pairs b > j (at least one)
   const int NT = blockDim.x;
   const int64_t npos = groups * 64;
   const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
@@ -471,25 +474,39 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
       // products.
       int xt = (int)threadIdx.x;
       asm volatile("" : "+v"(xt));
-      const int span = (int)max((int64_t)nj, LL);
-      for (int x0 = xt; x0 < span; x0 += SR * NT) {
-        RV sv[SR], kv[SR];
+      for (int x0 = xt, y0 = xt; x0 < nj || y0 < NPS; x0 += SR * NT, y0 += SV * NT) {
+        RV sv[SV], kv[SR];
 #pragma unroll
-        for (int u = 0; u < SR; ++u) {
-          const int x = x0 + u * NT;
-#pragma unroll
-          for (int g = 0; g < G; ++g) {
-            sv[u].v[g] = (has[g] && x < LL) ? srow[g][x] : T(0);
-            kv[u].v[g] = (has[g] && x < nj) ? __builtin_nontemporal_load(crow[g] + x) : T(0);
+        for (int u = 0; u < SV; ++u) {
+          // slice vector y is orbital pair (b, j), b > j, y = b(b-1)/2 + j: element b*L + j
+          // of the slice (the float root is exact to within one, corrected below)
+          const int y = y0 + u * NT;
+          const bool sl = y < NP;
+          int src = 0;
+          if (sl) {
+            int b = (int)((1.0f + __builtin_sqrtf(8.0f * (float)y + 1.0f)) * 0.5f);
+            if (b * (b - 1) / 2 > y) --b;
+            else if (b * (b + 1) / 2 <= y) ++b;
+            src = b * L + (y - b * (b - 1) / 2);
           }
+#pragma unroll
+          for (int g = 0; g < G; ++g) sv[u].v[g] = (has[g] && sl) ? srow[g][src] : T(0);
         }
 #pragma unroll
         for (int u = 0; u < SR; ++u) {
           const int x = x0 + u * NT;
-          if (x < LL) {
-            Vv[x] = sv[u];
-            if (sb2) Vv[sb2 + x] = sv[u];
-          }
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            kv[u].v[g] = (has[g] && x < nj) ? __builtin_nontemporal_load(crow[g] + x) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < SV; ++u) {
+          const int y = y0 + u * NT;
+          if (y < NPS) Vv[y] = sv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < SR; ++u) {
+          const int x = x0 + u * NT;
           if (x < nj) cbv[x] = kv[u];
         }
       }
@@ -684,7 +701,7 @@ void launch_ab_m(const SigmaPlan& P, size_t lds, const T* eri, const T* c, T* si
   if (!lds_at_zero) throw Error(SBDG_ERR_STATE, "alpha-beta kernel has static LDS: entry addresses invalid");
   hipLaunchKernelGGL((k_ab<T, M, NTMAX>), dim3((unsigned)nblk), dim3(E.nt), lds, st, P.na, P.nb, P.L, E.ntiles,
                      P.sa->rowptr.p, P.sa->col.p, P.sa->exc.p, E.soff.p, E.nq.p, E.ent.p, E.colperm.p, E.nch, E.chunk,
-                     E.sb2, E.kbase, E.groups, eri, c, sigma, assign);
+                     E.kbase, E.groups, eri, c, sigma, assign);
 }
 
 template <class T>

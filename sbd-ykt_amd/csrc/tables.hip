@@ -508,13 +508,12 @@ void ab_geometry(int64_t nb, int elem, int& nt, int& m) {
 // element size) in both the ket-row array and the integral array; equal addresses
 // broadcast and cost nothing.  Padding (the zero slot, pair 0) may be placed at any step
 // while a lane has slack.  Greedy: at each step the lanes with least slack choose first,
-// each taking the entry, and with a second slice copy (sb2 > 0) the copy, that adds the
-// fewest new addresses to slots already in use.  lists[l] holds lane l's entries (ket
-// byte address | pair << 18 | sign << 31); out gets K*64.  Slots count 16-byte LDS
-// vectors: the pair's slice vector is the pair index (sb2 + pair in the second copy), the
+// each taking the entry that adds the fewest new addresses to slots already in use.
+// lists[l] holds lane l's entries (ket byte address | pair << 18 | sign << 31); out gets
+// K*64.  Slots count 16-byte LDS vectors: the pair's slice vector is the pair index, the
 // ket's is its byte address / 16.
 static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int P, int S, uint32_t pad,
-                           uint32_t sb2, uint32_t* out) {
+                           uint32_t* out) {
   struct Use {
     uint32_t addr[8];
     int n;
@@ -541,22 +540,14 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
         const int slack = (K - e) - rem[l];
         auto& ls = lists[l];
         int best = -1, bc = 1 << 30;
-        uint32_t bv = 0;  // slice vector of the best entry
         for (int j = 0; j < rem[l]; ++j) {
           const uint32_t w = ls[j];
           const uint32_t ca = (w & 0x3FFFFu) >> 4;
-          uint32_t va = (w >> 18) & 0xFFFu;
-          int cv = cost1(uv[va % S], va);
-          if (sb2 && cv > 0) {
-            const uint32_t vb = va + sb2;
-            const int c2 = cost1(uv[vb % S], vb);
-            if (c2 < cv) cv = c2, va = vb;
-          }
-          const int cst = cost1(uc[ca % S], ca) + cv;
+          const uint32_t va = (w >> 18) & 0xFFFu;
+          const int cst = cost1(uc[ca % S], ca) + cost1(uv[va % S], va);
           if (cst < bc) {
             bc = cst;
             best = j;
-            bv = va;
             if (cst == 0) break;
           }
         }
@@ -567,7 +558,7 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
           if (best < 0 || cst < bc) best = -1;
         }
         if (best >= 0) {
-          w = (ls[best] & ~(0xFFFu << 18)) | (bv << 18);
+          w = ls[best];
           ls[best] = ls[rem[l] - 1];  // order of the rest does not matter
           --rem[l];
         }
@@ -583,7 +574,6 @@ static void schedule_block(std::vector<std::vector<uint32_t>>& lists, int K, int
 void build_ell(const Table& sb, int L, int64_t chunk, int prec_bytes, EllTable& e, hipStream_t st) {
   const int64_t nb = sb.n;
   if (L > 64) throw Error(SBDG_ERR_LIMIT, "norb beyond 64");
-  e.sb2 = ab_second_slice(L);
   e.kbase = ab_slice_span(L);
   if (16 * (e.kbase + chunk + 1) > (1 << 18)) throw Error(SBDG_ERR_LIMIT, "alpha-beta LDS beyond 256 KiB");
   e.chunk = chunk;
@@ -697,10 +687,10 @@ void build_ell(const Table& sb, int L, int64_t chunk, int prec_bytes, EllTable& 
           if (col[q] < j0 || col[q] >= j0 + chunk) continue;
           const uint32_t x = exc[q];
           const uint32_t j = x & 63u, b = (x >> 6) & 63u, neg = (x >> 24) & 1u;
-          lst.push_back(((kbase + (uint32_t)(col[q] - j0)) << 4) | ((b * (uint32_t)L + j) << 18) | (neg << 31));
+          lst.push_back(((kbase + (uint32_t)(col[q] - j0)) << 4) | (ab_pair(b, j) << 18) | (neg << 31));
         }
       }
-      schedule_block(lists, K[g * nch + k], P, S, pad, (uint32_t)e.sb2, ent.data() + base);
+      schedule_block(lists, K[g * nch + k], P, S, pad, ent.data() + base);
     }
   e.colperm.alloc(perm.size());
   SBDG_CHECK(hipMemcpy(e.colperm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));

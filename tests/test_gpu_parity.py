@@ -246,10 +246,10 @@ def test_sigma_c2_full_size_rows_and_symmetry():
 
 
 @pytest.mark.parametrize("L", [45, 46, 48])
-def test_sigma_slice_copies_vs_oracle(L):
-    """The alpha-beta kernel stages its integral slices twice (the schedule picks a copy per
-    entry) while an entry's 12-bit slice field reaches the second copy, up to norb=45, and
-    once from norb=46: sigma against the oracle on both sides of that limit."""
+def test_sigma_slice_pairs_vs_oracle(L):
+    """The alpha-beta kernel stages one integral slice vector per orbital pair b > j (990,
+    1,035 and 1,128 pairs, decoded from the pair index on the GPU) and reads (ai|bj) for
+    b < j from the pair (j, b): sigma against the oracle, which uses the full tensor."""
     case = workload_small(L, 4, 3, 300, 400, 61)
     s, p = solver(case), oproblem(case)
     x = np.random.default_rng(1).normal(size=s.ndet)
