@@ -963,6 +963,24 @@ int sbdg_set_integrals(sbdg_ctx* c, int norb, double core, const double* h1, con
       if (!std::isfinite(x)) return fail(c, SBDG_ERR_ARG, "non-finite one-electron integral");
     for (double x : c->eri)
       if (!std::isfinite(x)) return fail(c, SBDG_ERR_ARG, "non-finite two-electron integral");
+    if (layout == 0) {
+      // the kernels read (pq|rs) through its images ((ai|bj) for b < j from (ai|jb) in the
+      // alpha-beta kernel): the tensor must carry the 8-fold symmetry of real orbitals
+      double mx = 0.0;
+      for (double x : c->eri) mx = std::max(mx, std::fabs(x));
+      const double tol = 1e-10 * mx;
+      const double* e = c->eri.data();
+      auto at = [&](int64_t p, int64_t q, int64_t r, int64_t s_) { return e[((p * L + q) * L + r) * L + s_]; };
+      for (int64_t p = 0; p < L; ++p)
+        for (int64_t q = 0; q < L; ++q)
+          for (int64_t r = 0; r < L; ++r)
+            for (int64_t s_ = 0; s_ < L; ++s_) {
+              const double x = at(p, q, r, s_);
+              if (std::fabs(x - at(q, p, r, s_)) > tol || std::fabs(x - at(p, q, s_, r)) > tol ||
+                  std::fabs(x - at(r, s_, p, q)) > tol)
+                return fail(c, SBDG_ERR_ARG, "two-electron integrals lack the 8-fold symmetry (pq|rs) = (qp|rs) = (pq|sr) = (rs|pq)");
+            }
+    }
     c->have_ints = true;
     c->built = c->solved = c->have_rdm = c->have_dens = false;
     return SBDG_OK;

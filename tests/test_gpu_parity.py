@@ -621,6 +621,12 @@ def test_errors_from_the_host():
         solver(dict(case, bstr=np.array([0b111, 0b11], dtype=np.uint64)))
     with pytest.raises(sbdg.SbdgError):
         solver(case, max_nb=0)
+    # a full tensor without the 8-fold symmetry ((01|23) != (10|23) ...) is refused
+    bad = case["eri"].copy()
+    bad[0, 1, 2, 3] += 1e-3
+    with pytest.raises(sbdg.SbdgError) as ex:
+        solver(dict(case, eri=bad))
+    assert ex.value.code == sbdg.ERR_ARG and "symmetry" in str(ex.value)
     s = solver(case)
     with pytest.raises(sbdg.SbdgError) as ex:
         s.vector()
