@@ -100,8 +100,10 @@ const char *sbdg_last_error(const sbdg_ctx *ctx);
 int sbdg_default_options(sbdg_options *opt);
 
 /* Integrals, replacing SetupIntegrals (makeintegrals.h:10-55) on a parsed FCIDump.
- * h1: L*L row-major; eri_layout 0: full (pq|rs) row-major L^4 (8-fold symmetric);
- * eri_layout 1: the reference's packed store twoInt::store (integrals.h:24-67). */
+ * h1: L*L row-major; eri_layout 0: full (pq|rs) row-major L^4, 8-fold symmetric
+ * ((pq|rs) = (qp|rs) = (pq|sr) = (rs|pq) within 1e-10 of the largest element, else
+ * SBDG_ERR_ARG); eri_layout 1: the reference's packed store twoInt::store
+ * (integrals.h:24-67), symmetric by construction. */
 int sbdg_set_integrals(sbdg_ctx *ctx, int norb, double core, const double *h1, const double *eri,
                        int eri_layout);
 
