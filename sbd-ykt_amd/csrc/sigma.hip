@@ -467,8 +467,8 @@ __global__ void __launch_bounds__(NTMAX) k_ab(int64_t na, int64_t nb, int L, int
           srow[g] = eri + (int64_t)((int)((e >> 6) & 63u) * L + (int)(e & 63u)) * LL;  // (ai|..), a*L + i
         }
       }
-      // the G slices and the G ket rows' chunk into LDS: in rounds of SR positions per thread,
-      // every global load of a round issued before its LDS stores (the loads do not depend on
+      // the G slices and the G ket rows' chunk into LDS: in rounds of SR ket and SV slice
+      // vectors per thread, every global load of a round issued before its LDS stores (the loads do not depend on
       // each other), so a round costs one memory latency.  The thread index is opaque here so
       // that the addresses are formed per step rather than held in registers through the
       // products.
